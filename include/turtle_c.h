@@ -1,0 +1,164 @@
+/*
+ * turtle_c.h -- C ABI of the MI355X (gfx950) TurtleSort library
+ * (libturtle_hip.so).  Plain pointers, sizes and status codes; no HIP or
+ * torch types.  The C++ drop-in header include/turtle-sort.hpp is built on
+ * these calls, and so are the Python bindings.
+ *
+ * Each declaration names the reference interface it replaces
+ * (tugrul512bit/TurtleSort, file:line).
+ */
+#ifndef TURTLE_C_H
+#define TURTLE_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  The reference aborts the process on any CUDA error
+ * (TurtleGlobals::gpuErrchk, turtle-globals.cuh:18-30) and only prints
+ * device-side problems (turtle-sort.cu:146, 608, 860); every entry point here
+ * returns one of these instead. */
+typedef enum turtle_status {
+    TURTLE_OK = 0,
+    TURTLE_ERR_INVALID = 1,        /* null pointer, negative size, bad argument */
+    TURTLE_ERR_CAPACITY = 2,       /* n > max_n, or num_arrays*arr_size > max_n */
+    TURTLE_ERR_OOM = 3,            /* device or pinned allocation failed */
+    TURTLE_ERR_DTYPE = 4,          /* unknown dtype, or dtype differs from the handle's */
+    TURTLE_ERR_UNSUPPORTED = 5,    /* feature not available on this device */
+    TURTLE_ERR_DEVICE = 6,         /* HIP runtime error */
+    TURTLE_ERR_QUEUE_OVERFLOW = 7, /* device work queue / segment table full */
+    TURTLE_ERR_TIMEOUT = 8,        /* persistent kernel watchdog expired */
+    TURTLE_ERR_STATE = 9,          /* call out of order (Sync without StartSorting...) */
+    TURTLE_ERR_BUDGET = 10         /* request larger than the serving memory pool */
+} turtle_status;
+
+/* Key types.  The reference instantiates its kernels for char, short, int,
+ * long and their unsigned forms plus float/double (turtle-sort.cu:956-1174);
+ * long is 32-bit on its only platform (MSVC), so 64-bit keys are added here. */
+typedef enum turtle_dtype {
+    TURTLE_INT8 = 1, TURTLE_UINT8 = 2,
+    TURTLE_INT16 = 3, TURTLE_UINT16 = 4,
+    TURTLE_INT32 = 5, TURTLE_UINT32 = 6,
+    TURTLE_INT64 = 7, TURTLE_UINT64 = 8
+} turtle_dtype;
+
+typedef struct turtle_sorter *turtle_handle;
+
+/* Per-sort counters read back from the device after a sort. */
+typedef struct turtle_stats {
+    double device_ms;          /* device time of the last sort (HIP events) */
+    uint64_t n;                /* keys sorted */
+    uint64_t bytes_count;      /* algorithmic bytes of the count passes (key bytes read) */
+    uint64_t bytes_scatter;    /* algorithmic bytes of the partition scatters (read + write) */
+    uint64_t bytes_lds;        /* algorithmic bytes of the LDS sorts (read + write) */
+    uint64_t tasks_pivot, tasks_count, tasks_scatter, tasks_lds;
+    uint64_t segments;         /* partitioned segments created */
+    uint32_t max_depth;        /* deepest partition level reached */
+    uint32_t error;            /* device error flags of the last run (0 = none) */
+} turtle_stats;
+
+/* Tunables of the sort engine (chosen by the autotuner, or set by hand). */
+typedef struct turtle_tuning {
+    int32_t splitters;         /* pivots per partition: 3 (the reference's 3-pivot
+                                  partition) or 15 (two 3-pivot levels per pass) */
+    int32_t lds_cutoff;        /* segments at or below this length are sorted in LDS */
+    int32_t tiles_per_task;    /* queue granularity: tiles claimed per queue slot */
+    int32_t workgroups_per_cu; /* persistent grid = CUs x this (0 = occupancy limit) */
+} turtle_tuning;
+
+/* ---- sorter object: Turtle::TurtleSort<Type,TrackIndex> ------------------ */
+
+/* Replaces TurtleSort(int maxElements, bool optInCompression)
+ * (turtle-sort.cuh:63-113).  Allocates device buffers for max_n keys (and
+ * indices when track_index != 0).  compress is accepted; generic memory
+ * compression does not exist on this device, so the handle reports it off
+ * (turtle_compression_enabled) and creation still succeeds. */
+int turtle_create(int dtype, int64_t max_n, int track_index, int compress, turtle_handle *out);
+
+/* Replaces ~TurtleSort (turtle-sort.cuh:278-282) and the buffer frees of
+ * TurtleBuffer::~Buffer (turtle-buffer.cuh:82-99). */
+int turtle_destroy(turtle_handle h);
+
+/* Replaces MemoryCompressionSuccessful (turtle-sort.cuh:115-118); always 0. */
+int turtle_compression_enabled(turtle_handle h);
+
+/* Replaces StartSorting(std::vector<Type>*, std::vector<int>*)
+ * (turtle-sort.cuh:124-184).  keys/idx are host arrays of n elements; idx
+ * may be NULL.  Starts the host->device copy and the device sort on the
+ * handle's stream and returns.  The caller keeps both arrays alive and
+ * unchanged until turtle_sort_sync. */
+int turtle_sort_start(turtle_handle h, void *keys, int32_t *idx, int64_t n);
+
+/* Replaces Sync() (turtle-sort.cuh:188-258): waits, copies the sorted keys
+ * (and indices) back into the arrays given to turtle_sort_start and stores
+ * the wall-clock seconds since turtle_sort_start in *seconds (transfers
+ * included, as the reference). */
+int turtle_sort_sync(turtle_handle h, double *seconds);
+
+/* Replaces MultiSort<Type,ArrSize,BlockSize,UseSharedMemory>(numArrays,
+ * hostPtr) (turtle-sort.cuh:265-276 -> turtle-multi-sort.cuh:189-206):
+ * sorts num_arrays consecutive arrays of arr_size keys each, in place in the
+ * host buffer, synchronously; *seconds gets the wall time including
+ * transfers.  block_size and use_lds are accepted for source compatibility
+ * and do not change the result. */
+int turtle_multisort(turtle_handle h, int dtype, int64_t arr_size, int block_size, int use_lds,
+                     int64_t num_arrays, void *host_data, double *seconds);
+
+/* ---- device-resident forms (inputs already in HBM) ---------------------- */
+
+/* Device half of StartSorting+Sync: replaces the wave chain
+ * copyTasksBack -> resetNumTasks -> quickSortWithoutStreamCompaction/bruteSort
+ * (turtle-sort.cu:35-188, 573-934) and the chunk merge of Sync
+ * (turtle-sort.cu:190-300, turtle-sort.cuh:192-243) with one persistent
+ * kernel draining a device work queue.  Sorts keys_in[0..n) into keys_out
+ * (and carries idx_in into idx_out when both are non-NULL; the handle must
+ * track indices).  keys_in is not modified; keys_out may equal keys_in.
+ * stream is a hipStream_t (NULL = the handle's stream).  Asynchronous. */
+int turtle_sort_device(turtle_handle h, const void *keys_in, const int32_t *idx_in,
+                       void *keys_out, int32_t *idx_out, int64_t n, void *stream);
+
+/* Batched uneven segments in one persistent launch (no reference
+ * counterpart; the reference's only route is a loop of StartSorting+Sync).
+ * Segment s is [offsets[s], offsets[s+1]) of the concatenated arrays;
+ * offsets is a HOST array of num_segments+1 entries.  Asynchronous. */
+int turtle_segmented_sort_device(turtle_handle h, const void *keys_in, const int32_t *idx_in,
+                                 void *keys_out, int32_t *idx_out,
+                                 const int64_t *offsets, int64_t num_segments, void *stream);
+
+/* Device half of MultiSort (turtle-multi-sort.cuh:86-182): num_arrays
+ * consecutive arrays of arr_size keys, keys_in -> keys_out (may alias).
+ * Asynchronous. */
+int turtle_multisort_device(turtle_handle h, int dtype, int64_t arr_size,
+                            const void *keys_in, void *keys_out, int64_t num_arrays, void *stream);
+
+/* Waits for the handle's last device work and fills *st (may be NULL). */
+int turtle_synchronize(turtle_handle h, turtle_stats *st);
+
+/* Tuning of a handle (defaults are the autotuned table, or built-ins). */
+int turtle_get_tuning(turtle_handle h, turtle_tuning *t);
+int turtle_set_tuning(turtle_handle h, const turtle_tuning *t);
+
+/* ---- synthetic inputs on the device (benchmarks and tests) -------------- */
+
+/* keys[i] = (splitmix64 draw i of seed) >> shift, truncated to the dtype
+ * width; draw i is mix(seed + (i+1) * 0x9e3779b97f4a7c15). */
+int turtle_fill_keys(int dtype, void *keys, int64_t n, uint64_t seed, int shift, void *stream);
+/* idx[i] = i */
+int turtle_fill_iota(int32_t *idx, int64_t n, void *stream);
+
+/* ---- misc ---------------------------------------------------------------- */
+const char *turtle_status_string(int status);
+/* Last error message recorded by this thread (empty string if none). */
+const char *turtle_last_error(void);
+int turtle_dtype_size(int dtype);
+/* Number of compute units of the current device (0 if no device). */
+int turtle_device_cus(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TURTLE_C_H */

@@ -1,0 +1,138 @@
+"""Parity of the plain sort (StartSorting/Sync and the device-resident form)
+with the CPU oracle.  Keys must be bit-identical to the oracle's; the GPU
+path is stable, so the index output must equal a stable sort's (numpy
+argsort kind="stable"), which also proves the (key, idx) multiset is kept."""
+import numpy as np
+import pytest
+
+from turtlesort_amd import TurtleSort, workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+DTYPES = [np.int8, np.uint8, np.int16, np.uint16, np.int32, np.uint32, np.int64, np.uint64]
+EDGE_SIZES = [0, 1, 2, 31, 32, 33, 127, 128, 129, 4095, 4096, 4097, 8191, 8192, 8193, 65537]
+
+
+def make_keys(dtype, n, dist, seed):
+    dt = np.dtype(dtype)
+    bits = 8 * dt.itemsize
+    if dist == "full":
+        k = wl.gen_keys(dt, n, seed, 0)
+    elif dist == "dup15":
+        k = wl.gen_keys(dt, n, seed, 64 - min(15, bits - (1 if dt.kind == "i" else 0)))
+    elif dist == "few16":
+        k = wl.gen_keys(dt, n, seed, 60)
+    elif dist == "asc":
+        k = np.sort(wl.gen_keys(dt, n, seed, 0), kind="stable")
+    elif dist == "desc":
+        k = np.sort(wl.gen_keys(dt, n, seed, 0), kind="stable")[::-1].copy()
+    elif dist == "equal":
+        k = np.full(n, wl.gen_keys(dt, 1, seed, 0)[0], dtype=dt)
+    elif dist == "extremes":
+        info = np.iinfo(dt)
+        k = np.where(wl.gen_keys(np.uint8, n, seed, 0) & 1, info.min, info.max).astype(dt)
+    elif dist == "organ":
+        h = wl.gen_keys(dt, (n + 1) // 2, seed, 0)
+        h.sort()
+        k = np.concatenate([h, h[::-1]])[:n].copy()
+    else:
+        raise ValueError(dist)
+    return np.ascontiguousarray(k)
+
+
+def check_against_oracle(orc, keys, out_keys, out_idx):
+    ok, oi = orc.turtle_sort(keys, np.arange(keys.size, dtype=np.int32))
+    assert out_keys.dtype == keys.dtype
+    np.testing.assert_array_equal(out_keys, ok)              # bit-exact keys
+    # the oracle's index order among equal keys is one valid schedule of the
+    # reference; compare the (key, idx) multisets
+    assert np.array_equal(np.sort(oi), np.arange(keys.size))
+    np.testing.assert_array_equal(keys[oi], ok)
+    if out_idx is not None:
+        stable = np.argsort(keys, kind="stable").astype(np.int32)
+        np.testing.assert_array_equal(out_idx, stable)         # stable => exact
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("dist", ["full", "few16", "asc", "desc", "equal", "extremes", "dup15", "organ"])
+def test_start_sorting_edges(orc, torch_dev, dtype, dist):
+    sorter = TurtleSort(dtype, 70000)
+    for i, n in enumerate(EDGE_SIZES):
+        print(f"case dtype={np.dtype(dtype)} dist={dist} n={n}", flush=True)
+        keys = make_keys(dtype, n, dist, 1000 + i)
+        k = keys.copy()
+        idx = np.arange(n, dtype=np.int32)
+        sorter.start_sorting(k, idx)
+        sec = sorter.sync()
+        assert sec >= 0
+        check_against_oracle(orc, keys, k, idx)
+    sorter.close()
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64, np.uint8])
+def test_keys_only(orc, torch_dev, dtype):
+    sorter = TurtleSort(dtype, 300000)
+    for n in [0, 1, 100, 8193, 300000]:
+        keys = make_keys(dtype, n, "full", 77 + n)
+        k = keys.copy()
+        sorter.start_sorting(k)
+        sorter.sync()
+        ok, _ = orc.turtle_sort(keys)
+        np.testing.assert_array_equal(k, ok)
+    sorter.close()
+
+
+def test_config1_heavy_duplicates(orc, torch_dev):
+    """n = 1,048,576 int32 keys in [0, 32767], idx = iota, via StartSorting."""
+    n = wl.C1_N
+    keys = wl.gen_keys(np.int32, n, wl.SEED_C1, wl.SHIFT_15BIT)
+    assert keys.min() >= 0 and keys.max() <= 32767
+    sorter = TurtleSort(np.int32, n)
+    k = keys.copy()
+    idx = np.arange(n, dtype=np.int32)
+    sorter.start_sorting(k, idx)
+    sorter.sync()
+    check_against_oracle(orc, keys, k, idx)
+    sorter.close()
+
+
+@pytest.mark.parametrize("dist", ["full", "few16", "asc", "desc"])
+def test_medium_sizes(orc, torch_dev, dist):
+    sorter = TurtleSort(np.int32, 2_000_000)
+    for n in [200_001, 2_000_000]:
+        keys = make_keys(np.int32, n, dist, 5 + n)
+        k = keys.copy()
+        idx = np.arange(n, dtype=np.int32)
+        sorter.start_sorting(k, idx)
+        sorter.sync()
+        check_against_oracle(orc, keys, k, idx)
+    sorter.close()
+
+
+def test_device_form_repeat_and_inplace(torch_dev):
+    """Same input twice gives identical output (determinism); in-place works."""
+    torch = torch_dev
+    n = 3_000_000
+    sorter = TurtleSort(np.int32, n)
+    kin = torch.empty(n, dtype=torch.int32, device="cuda")
+    from turtlesort_amd import fill_keys, fill_iota
+    fill_keys(kin, 99, 33)
+    iin = torch.empty(n, dtype=torch.int32, device="cuda")
+    fill_iota(iin)
+    outs = []
+    for _ in range(2):
+        ko = torch.empty_like(kin)
+        io = torch.empty_like(iin)
+        sorter.sort_device(kin, ko, iin, io)
+        sorter.synchronize()
+        outs.append((ko, io))
+    assert torch.equal(outs[0][0], outs[1][0]) and torch.equal(outs[0][1], outs[1][1])
+    ref_k, ref_i = torch.sort(kin.to(torch.int64), stable=True)
+    assert torch.equal(outs[0][0].to(torch.int64), ref_k)
+    assert torch.equal(outs[0][1].to(torch.int64), ref_i)
+    k2 = kin.clone()
+    i2 = iin.clone()
+    sorter.sort_device(k2, k2, i2, i2)
+    sorter.synchronize()
+    assert torch.equal(k2, outs[0][0]) and torch.equal(i2, outs[0][1])
+    sorter.close()

@@ -1,0 +1,10 @@
+"""TurtleSort for AMD Instinct MI355X (gfx950).
+
+The sort runs in hand-written HIP kernels in libturtle_hip.so (built in-tree
+from turtlesort_amd/csrc); this package is the Python mirror of the
+reference's Turtle::TurtleSort API over that library's C ABI.
+"""
+from ._lib import TurtleError, load  # noqa: F401
+from .api import TurtleSort, fill_iota, fill_keys, dtype_code  # noqa: F401
+
+__all__ = ["TurtleSort", "TurtleError", "fill_keys", "fill_iota", "dtype_code", "load"]

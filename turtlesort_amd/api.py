@@ -1,0 +1,178 @@
+"""Python mirror of the reference's Turtle::TurtleSort API over the C ABI.
+
+Names and argument meaning follow turtle-sort.cuh:
+  TurtleSort(dtype, max_elements, compress=False, track_index=True)  (63-113)
+  .start_sorting(keys, idx=None)   StartSorting (124-184), host numpy arrays,
+                                   sorted in place by .sync()
+  .sync() -> seconds               Sync (188-258), wall time incl. transfers
+  .multi_sort(arr_size, data, block_size=32, use_shared_memory=False)
+                                   MultiSort (265-276), in place, -> seconds
+  .memory_compression_successful() MemoryCompressionSuccessful (115-118)
+plus device-resident forms that take torch tensors already on the GPU.
+Errors raise TurtleError instead of exiting the process (the reference calls
+exit() from gpuErrchk, turtle-globals.cuh:18-30).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+_NP_TO_CODE = {
+    np.dtype(np.int8): _lib.INT8, np.dtype(np.uint8): _lib.UINT8,
+    np.dtype(np.int16): _lib.INT16, np.dtype(np.uint16): _lib.UINT16,
+    np.dtype(np.int32): _lib.INT32, np.dtype(np.uint32): _lib.UINT32,
+    np.dtype(np.int64): _lib.INT64, np.dtype(np.uint64): _lib.UINT64,
+}
+CODE_TO_NP = {v: k for k, v in _NP_TO_CODE.items()}
+NAME_TO_CODE = {str(k): v for k, v in _NP_TO_CODE.items()}
+
+
+def dtype_code(dtype):
+    """numpy dtype / name / torch dtype / code -> turtle dtype code."""
+    if isinstance(dtype, int):
+        if dtype not in CODE_TO_NP:
+            raise ValueError(f"unknown dtype code {dtype}")
+        return dtype
+    try:
+        import torch
+        if isinstance(dtype, torch.dtype):
+            m = {torch.int8: _lib.INT8, torch.uint8: _lib.UINT8, torch.int16: _lib.INT16,
+                 torch.int32: _lib.INT32, torch.int64: _lib.INT64}
+            for name in ("uint16", "uint32", "uint64"):
+                td = getattr(torch, name, None)
+                if td is not None:
+                    m[td] = NAME_TO_CODE[name]
+            return m[dtype]
+    except ImportError:
+        pass
+    return _NP_TO_CODE[np.dtype(dtype)]
+
+
+def _ptr(t):
+    """device pointer of a torch tensor, or None"""
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    """hipStream_t to run on: the given one, else torch's current stream, so
+    that device work is ordered after the torch ops that produced the inputs"""
+    if stream is not None:
+        return ctypes.c_void_p(stream)
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class TurtleSort:
+    def __init__(self, dtype, max_elements, compress=False, track_index=True):
+        self._lib = _lib.load()
+        self.dtype = dtype_code(dtype)
+        self.np_dtype = CODE_TO_NP[self.dtype]
+        self.max_n = int(max_elements)
+        self.track_index = bool(track_index)
+        h = ctypes.c_void_p()
+        _lib.check(self._lib.turtle_create(self.dtype, self.max_n, int(self.track_index),
+                                           int(bool(compress)), ctypes.byref(h)))
+        self._h = h
+        self._pending = None
+
+    # ---- lifetime --------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.turtle_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- reference API ------------------------------------------------------------
+    def memory_compression_successful(self):
+        return bool(self._lib.turtle_compression_enabled(self._h))
+
+    def start_sorting(self, keys, idx=None):
+        if not isinstance(keys, np.ndarray) or keys.dtype != self.np_dtype or not keys.flags.c_contiguous:
+            raise TypeError(f"keys must be a C-contiguous numpy array of {self.np_dtype}")
+        if idx is not None:
+            if not isinstance(idx, np.ndarray) or idx.dtype != np.int32 or not idx.flags.c_contiguous:
+                raise TypeError("idx must be a C-contiguous numpy int32 array")
+            if idx.shape[0] != keys.shape[0]:
+                raise ValueError("keys and idx must have the same length")
+        _lib.check(self._lib.turtle_sort_start(
+            self._h, keys.ctypes.data_as(ctypes.c_void_p),
+            None if idx is None else idx.ctypes.data_as(ctypes.c_void_p), keys.shape[0]))
+        self._pending = (keys, idx)   # keep the arrays alive until sync()
+
+    def sync(self):
+        sec = ctypes.c_double(0.0)
+        try:
+            _lib.check(self._lib.turtle_sort_sync(self._h, ctypes.byref(sec)))
+        finally:
+            self._pending = None
+        return sec.value
+
+    def multi_sort(self, arr_size, data, block_size=32, use_shared_memory=False):
+        if not isinstance(data, np.ndarray) or data.dtype != self.np_dtype or not data.flags.c_contiguous:
+            raise TypeError(f"data must be a C-contiguous numpy array of {self.np_dtype}")
+        if arr_size <= 0 or data.size % arr_size:
+            raise ValueError("data length must be a multiple of arr_size")
+        sec = ctypes.c_double(0.0)
+        _lib.check(self._lib.turtle_multisort(
+            self._h, self.dtype, int(arr_size), int(block_size), int(bool(use_shared_memory)),
+            data.size // arr_size, data.ctypes.data_as(ctypes.c_void_p), ctypes.byref(sec)))
+        return sec.value
+
+    # ---- device-resident forms (torch tensors on the GPU) --------------------------
+    def sort_device(self, keys_in, keys_out, idx_in=None, idx_out=None, stream=None):
+        n = keys_in.numel()
+        _lib.check(self._lib.turtle_sort_device(
+            self._h, _ptr(keys_in), _ptr(idx_in), _ptr(keys_out), _ptr(idx_out), n, _stream(stream)))
+
+    def segmented_sort_device(self, keys_in, keys_out, offsets, idx_in=None, idx_out=None, stream=None):
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        _lib.check(self._lib.turtle_segmented_sort_device(
+            self._h, _ptr(keys_in), _ptr(idx_in), _ptr(keys_out), _ptr(idx_out),
+            off.ctypes.data_as(ctypes.c_void_p), off.size - 1, _stream(stream)))
+
+    def multi_sort_device(self, arr_size, keys_in, keys_out, stream=None):
+        num = keys_in.numel() // arr_size
+        _lib.check(self._lib.turtle_multisort_device(
+            self._h, self.dtype, int(arr_size), _ptr(keys_in), _ptr(keys_out), num, _stream(stream)))
+
+    def synchronize(self):
+        st = _lib.Stats()
+        _lib.check(self._lib.turtle_synchronize(self._h, ctypes.byref(st)))
+        return st.as_dict()
+
+    # ---- tuning -------------------------------------------------------------------
+    def get_tuning(self):
+        t = _lib.Tuning()
+        _lib.check(self._lib.turtle_get_tuning(self._h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, **kw):
+        t = _lib.Tuning()
+        _lib.check(self._lib.turtle_get_tuning(self._h, ctypes.byref(t)))
+        for k, v in kw.items():
+            setattr(t, k, int(v))
+        _lib.check(self._lib.turtle_set_tuning(self._h, ctypes.byref(t)))
+
+
+def fill_keys(keys, seed, shift, stream=None):
+    """keys[i] = splitmix64 draw i of seed >> shift (device tensor, in place)."""
+    lib = _lib.load()
+    _lib.check(lib.turtle_fill_keys(dtype_code(keys.dtype), _ptr(keys), keys.numel(), int(seed) & (2**64 - 1),
+                                    int(shift), _stream(stream)))
+
+
+def fill_iota(idx, stream=None):
+    lib = _lib.load()
+    _lib.check(lib.turtle_fill_iota(_ptr(idx), idx.numel(), _stream(stream)))

@@ -1,0 +1,605 @@
+// ts_capi.hip -- the C ABI of libturtle_hip.so (declared in include/turtle_c.h).
+//
+// A handle is the device state of one Turtle::TurtleSort<Type,TrackIndex>
+// object (turtle-sort.cuh:31-113): key/index buffers sized for max_n, the
+// engine's control block, work queue, segment and tile tables, and one HIP
+// stream.  Status codes replace the reference's print-and-exit error policy
+// (turtle-globals.cuh:18-30).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+#include "../../include/turtle_c.h"
+#include "ts_engine.h"
+#include "ts_internal.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+int fail_hip(hipError_t e, const char *what)
+{
+    g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? TURTLE_ERR_OOM : TURTLE_ERR_DEVICE;
+}
+
+#define HIPCHK(call, what)                                 \
+    do {                                                   \
+        hipError_t e_ = (call);                            \
+        if (e_ != hipSuccess) return fail_hip(e_, what);   \
+    } while (0)
+
+int dtype_bytes(int dtype)
+{
+    switch (dtype) {
+    case TURTLE_INT8: case TURTLE_UINT8: return 1;
+    case TURTLE_INT16: case TURTLE_UINT16: return 2;
+    case TURTLE_INT32: case TURTLE_UINT32: return 4;
+    case TURTLE_INT64: case TURTLE_UINT64: return 8;
+    default: return 0;
+    }
+}
+
+bool dtype_signed(int dtype)
+{
+    return dtype == TURTLE_INT8 || dtype == TURTLE_INT16 || dtype == TURTLE_INT32 || dtype == TURTLE_INT64;
+}
+
+uint64_t flip_of(int dtype)
+{
+    return dtype_signed(dtype) ? (1ull << (8 * dtype_bytes(dtype) - 1)) : 0ull;
+}
+
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+uint64_t env_u64(const char *name, uint64_t dflt)
+{
+    const char *v = getenv(name);
+    if (!v || !*v) return dflt;
+    return strtoull(v, nullptr, 10);
+}
+
+}  // namespace
+
+struct turtle_sorter {
+    int dtype = 0, kb = 0;
+    uint64_t flip = 0;
+    int track = 0;
+    int64_t max_n = 0;
+    int device = 0;
+    int cus = 0;
+    hipStream_t stream = nullptr;
+    hipStream_t last_stream = nullptr;
+    // key / index buffers of max_n elements
+    void *kin = nullptr, *kout = nullptr, *kt0 = nullptr, *kt1 = nullptr;
+    uint32_t *iin = nullptr, *iout = nullptr, *it0 = nullptr, *it1 = nullptr;
+    // engine tables
+    ts::Ctrl *ctrl = nullptr;
+    ts::Slot *q = nullptr;
+    ts::Seg *segs = nullptr;
+    uint32_t *tiles = nullptr;
+    uint32_t qcap = 0, segcap = 0, tilecap = 0;
+    uint32_t *dev_off = nullptr;
+    size_t dev_off_cap = 0;
+    std::vector<uint32_t> host_off;
+    turtle_tuning tune{};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timed = false;
+    uint64_t last_n = 0;
+    // StartSorting / Sync state
+    void *host_keys = nullptr;
+    int32_t *host_idx = nullptr;
+    int64_t host_n = 0;
+    bool started = false;
+    double t_start = 0.0;
+    // what the last engine run needs to be run again after its tables grow
+    bool last_engine = false;
+    uint64_t last_total = 0;
+    uint32_t last_nroots = 0;
+};
+
+namespace {
+
+void free_tables(turtle_sorter *h)
+{
+    if (h->q) (void)hipFree(h->q);
+    if (h->segs) (void)hipFree(h->segs);
+    if (h->tiles) (void)hipFree(h->tiles);
+    h->q = nullptr;
+    h->segs = nullptr;
+    h->tiles = nullptr;
+    h->qcap = h->segcap = h->tilecap = 0;
+}
+
+// Sizes the queue, segment and tile tables for a run over `total` keys in
+// `nroots` root segments.  Bounds: segments longer than the cutoff are
+// disjoint at each depth, so about (total/cutoff) per depth; tiles are
+// ceil(len/TILE) per segment; tasks are one pivot, count and scatter tasks,
+// and at most S+1 children per segment.  `scale` grows them after an
+// overflow.  Table memory is reused while large enough.
+int ensure_tables(turtle_sorter *h, uint64_t total, uint32_t nroots, uint32_t scale)
+{
+    const uint64_t cutoff = (uint64_t)h->tune.lds_cutoff;
+    const uint64_t tile = ts::engine_tile(h->kb);
+    const uint64_t S = (uint64_t)h->tune.splitters;
+    uint64_t segcap = (4 * (total / cutoff + 1) + 1024) * scale + nroots;
+    uint64_t tilecap = (8 * (total / tile + 1)) * scale + segcap;
+    uint64_t qcap = segcap * (S + 2) + 2 * tilecap + nroots + 1024;
+    const uint64_t cap_q = env_u64("TURTLE_QUEUE_CAP", 0);
+    if (cap_q) qcap = cap_q;
+    if (qcap > 0xffffff00ull || segcap > 0xffffff00ull || tilecap > 0xffffff00ull)
+        return fail(TURTLE_ERR_CAPACITY, "work tables would exceed 2^32 entries");
+    if (h->q && h->qcap >= qcap && h->segcap >= segcap && h->tilecap >= tilecap && !cap_q) return TURTLE_OK;
+    if (h->q && cap_q && h->qcap == qcap && h->segcap >= segcap && h->tilecap >= tilecap) return TURTLE_OK;
+    // the old tables may still be in use by a run in flight
+    if (h->q) HIPCHK(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    free_tables(h);
+    HIPCHK(hipMalloc((void **)&h->q, qcap * sizeof(ts::Slot)), "hipMalloc(queue)");
+    HIPCHK(hipMemset(h->q, 0, qcap * sizeof(ts::Slot)), "hipMemset(queue)");
+    HIPCHK(hipMalloc((void **)&h->segs, segcap * sizeof(ts::Seg)), "hipMalloc(segments)");
+    HIPCHK(hipMalloc((void **)&h->tiles, tilecap * ts::MAX_BINS * sizeof(uint32_t)), "hipMalloc(tiles)");
+    h->qcap = (uint32_t)qcap;
+    h->segcap = (uint32_t)segcap;
+    h->tilecap = (uint32_t)tilecap;
+    // the queue was re-created with gen 0 everywhere; the control block keeps
+    // counting generations, which start above 0 after the first reset
+    return TURTLE_OK;
+}
+
+int grid_of(turtle_sorter *h, bool idx)
+{
+    int occ = ts::engine_occupancy(h->kb, idx, h->tune.splitters);
+    int per_cu = h->tune.workgroups_per_cu > 0 && h->tune.workgroups_per_cu < occ ? h->tune.workgroups_per_cu : occ;
+    int g = h->cus * per_cu;
+    return g > 0 ? g : 1;
+}
+
+ts::Params make_params(turtle_sorter *h, const void *kin, const uint32_t *iin, void *kout, uint32_t *iout)
+{
+    ts::Params p{};
+    p.kin = kin;
+    p.iin = iin;
+    p.kout = kout;
+    p.iout = iout;
+    p.kt0 = h->kt0;
+    p.kt1 = h->kt1;
+    p.it0 = h->it0;
+    p.it1 = h->it1;
+    p.ctrl = h->ctrl;
+    p.q = h->q;
+    p.segs = h->segs;
+    p.tiles = h->tiles;
+    p.qcap = h->qcap;
+    p.segcap = h->segcap;
+    p.tilecap = h->tilecap;
+    p.cutoff = (uint32_t)h->tune.lds_cutoff;
+    p.tiles_per_task = (uint32_t)(h->tune.tiles_per_task > 0 ? h->tune.tiles_per_task : 1);
+    p.flip = h->flip;
+    const uint64_t ms = env_u64("TURTLE_WATCHDOG_MS", 20000);
+    p.timeout_ticks = ms * 100000ull;   // wall_clock64 runs at 100 MHz
+    return p;
+}
+
+hipStream_t stream_of(turtle_sorter *h, void *stream)
+{
+    return stream ? (hipStream_t)stream : h->stream;
+}
+
+// One engine run over device buffers.  In-place requests (kin == kout) first
+// copy the input into the second ping-pong buffer, which depth 0 only reads.
+int run_engine(turtle_sorter *h, const void *kin, const uint32_t *iin, void *kout, uint32_t *iout, uint64_t total,
+               const uint32_t *dev_off, uint32_t nseg, hipStream_t st)
+{
+    const bool idx = iin != nullptr && iout != nullptr;
+    int rc = ensure_tables(h, total, nseg, 1);
+    if (rc) return rc;
+    if (total > 0 && kin == kout) {
+        HIPCHK(hipMemcpyAsync(h->kt1, kin, total * h->kb, hipMemcpyDeviceToDevice, st), "hipMemcpyAsync(in-place keys)");
+        kin = h->kt1;
+    }
+    if (idx && total > 0 && (const void *)iin == (const void *)iout) {
+        HIPCHK(hipMemcpyAsync(h->it1, iin, total * 4, hipMemcpyDeviceToDevice, st), "hipMemcpyAsync(in-place idx)");
+        iin = h->it1;
+    }
+    ts::Params p = make_params(h, kin, idx ? iin : nullptr, kout, idx ? iout : nullptr);
+    HIPCHK(hipEventRecord(h->ev0, st), "hipEventRecord");
+    HIPCHK(ts::engine_run(p, h->kb, idx, h->tune.splitters, grid_of(h, idx), dev_off, nseg, (uint32_t)total, st),
+           "engine launch");
+    HIPCHK(hipEventRecord(h->ev1, st), "hipEventRecord");
+    h->timed = true;
+    h->last_stream = st;
+    h->last_n = total;
+    h->last_engine = true;
+    h->last_total = total;
+    h->last_nroots = nseg;
+    return TURTLE_OK;
+}
+
+int read_stats(turtle_sorter *h, turtle_stats *st, uint32_t *err_out)
+{
+    ts::Ctrl c;
+    HIPCHK(hipMemcpy(&c, h->ctrl, sizeof(c), hipMemcpyDeviceToHost), "hipMemcpy(ctrl)");
+    float ms = 0.f;
+    if (h->timed) HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1), "hipEventElapsedTime");
+    if (st) {
+        memset(st, 0, sizeof(*st));
+        st->device_ms = ms;
+        st->n = h->last_n;
+        if (h->last_engine) {
+            st->bytes_count = c.bytes_count;
+            st->bytes_scatter = c.bytes_scatter;
+            st->bytes_lds = c.bytes_lds;
+            st->tasks_pivot = c.tasks[ts::T_PIVOT];
+            st->tasks_count = c.tasks[ts::T_COUNT];
+            st->tasks_scatter = c.tasks[ts::T_SCATTER];
+            st->tasks_lds = c.tasks[ts::T_LDS];
+            st->segments = c.seg_alloc;
+            st->max_depth = c.max_depth;
+            st->error = c.error;
+        }
+    }
+    *err_out = h->last_engine ? c.error : 0u;
+    return TURTLE_OK;
+}
+
+int error_status(uint32_t err)
+{
+    if (err == 0) return TURTLE_OK;
+    if (err & ts::ERR_TIMEOUT) return fail(TURTLE_ERR_TIMEOUT, "persistent sort kernel watchdog expired");
+    if (err & (ts::ERR_QUEUE | ts::ERR_SEGS | ts::ERR_TILES))
+        return fail(TURTLE_ERR_QUEUE_OVERFLOW, "device work queue or segment/tile table overflow");
+    return fail(TURTLE_ERR_DEVICE, "inconsistent partition detected on the device");
+}
+
+bool valid_tuning(const turtle_tuning &t, int kb)
+{
+    if (!ts::engine_supports(t.splitters)) return false;
+    if (t.lds_cutoff < 16 || (uint32_t)t.lds_cutoff > ts::engine_cap(kb)) return false;
+    if (t.tiles_per_task < 1 || t.tiles_per_task > 1024) return false;
+    if (t.workgroups_per_cu < 0 || t.workgroups_per_cu > 64) return false;
+    return true;
+}
+
+turtle_tuning default_tuning(int kb)
+{
+    turtle_tuning t;
+    t.splitters = 3;
+    t.lds_cutoff = (int32_t)ts::engine_cap(kb);
+    t.tiles_per_task = 1;
+    t.workgroups_per_cu = 0;
+    return t;
+}
+
+int alloc_buf(void **p, size_t bytes, const char *what)
+{
+    HIPCHK(hipMalloc(p, bytes ? bytes : 16), what);
+    return TURTLE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *turtle_status_string(int s)
+{
+    switch (s) {
+    case TURTLE_OK: return "ok";
+    case TURTLE_ERR_INVALID: return "invalid argument";
+    case TURTLE_ERR_CAPACITY: return "capacity exceeded";
+    case TURTLE_ERR_OOM: return "out of memory";
+    case TURTLE_ERR_DTYPE: return "bad dtype";
+    case TURTLE_ERR_UNSUPPORTED: return "unsupported";
+    case TURTLE_ERR_DEVICE: return "device error";
+    case TURTLE_ERR_QUEUE_OVERFLOW: return "work queue overflow";
+    case TURTLE_ERR_TIMEOUT: return "watchdog timeout";
+    case TURTLE_ERR_STATE: return "bad call order";
+    case TURTLE_ERR_BUDGET: return "memory budget exceeded";
+    default: return "unknown status";
+    }
+}
+
+const char *turtle_last_error(void) { return g_last_error.c_str(); }
+
+int turtle_dtype_size(int dtype) { return dtype_bytes(dtype); }
+
+int turtle_device_cus(void)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    hipDeviceProp_t pr;
+    if (hipGetDeviceProperties(&pr, dev) != hipSuccess) return 0;
+    return pr.multiProcessorCount;
+}
+
+int turtle_create(int dtype, int64_t max_n, int track_index, int compress, turtle_handle *out)
+{
+    if (!out) return fail(TURTLE_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    const int kb = dtype_bytes(dtype);
+    if (!kb) return fail(TURTLE_ERR_DTYPE, "unknown dtype");
+    if (max_n < 0 || max_n >= (int64_t)1 << 31) return fail(TURTLE_ERR_INVALID, "max_n must be in [0, 2^31)");
+    (void)compress;   // generic compressible memory is an NVIDIA-only feature
+    turtle_sorter *h = new turtle_sorter();
+    h->dtype = dtype;
+    h->kb = kb;
+    h->flip = flip_of(dtype);
+    h->track = track_index ? 1 : 0;
+    h->max_n = max_n;
+    h->tune = default_tuning(kb);
+    int rc = TURTLE_OK;
+    hipDeviceProp_t pr;
+    hipError_t e = hipGetDevice(&h->device);
+    if (e == hipSuccess) e = hipGetDeviceProperties(&pr, h->device);
+    if (e != hipSuccess) {
+        delete h;
+        return fail_hip(e, "no HIP device");
+    }
+    h->cus = pr.multiProcessorCount;
+    const size_t kbytes = (size_t)max_n * kb, ibytes = (size_t)max_n * 4;
+    if ((rc = alloc_buf(&h->kin, kbytes, "hipMalloc(keys in)")) || (rc = alloc_buf(&h->kout, kbytes, "hipMalloc(keys out)")) ||
+        (rc = alloc_buf(&h->kt0, kbytes, "hipMalloc(keys tmp0)")) || (rc = alloc_buf(&h->kt1, kbytes, "hipMalloc(keys tmp1)"))) {
+        turtle_destroy(h);
+        return rc;
+    }
+    if (h->track) {
+        if ((rc = alloc_buf((void **)&h->iin, ibytes, "hipMalloc(idx in)")) ||
+            (rc = alloc_buf((void **)&h->iout, ibytes, "hipMalloc(idx out)")) ||
+            (rc = alloc_buf((void **)&h->it0, ibytes, "hipMalloc(idx tmp0)")) ||
+            (rc = alloc_buf((void **)&h->it1, ibytes, "hipMalloc(idx tmp1)"))) {
+            turtle_destroy(h);
+            return rc;
+        }
+    }
+    if ((e = hipMalloc((void **)&h->ctrl, sizeof(ts::Ctrl))) != hipSuccess ||
+        (e = hipMemset(h->ctrl, 0, sizeof(ts::Ctrl))) != hipSuccess ||
+        (e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreate(&h->ev0)) != hipSuccess || (e = hipEventCreate(&h->ev1)) != hipSuccess) {
+        turtle_destroy(h);
+        return fail_hip(e, "turtle_create");
+    }
+    h->last_stream = h->stream;
+    if ((rc = ensure_tables(h, (uint64_t)max_n, 1, 1))) {
+        turtle_destroy(h);
+        return rc;
+    }
+    if ((e = hipDeviceSynchronize()) != hipSuccess) {
+        turtle_destroy(h);
+        return fail_hip(e, "turtle_create sync");
+    }
+    *out = h;
+    return TURTLE_OK;
+}
+
+int turtle_destroy(turtle_handle h)
+{
+    if (!h) return TURTLE_OK;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    void *bufs[] = {h->kin, h->kout, h->kt0, h->kt1, h->iin, h->iout, h->it0, h->it1, h->ctrl, h->dev_off};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    free_tables(h);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return TURTLE_OK;
+}
+
+int turtle_compression_enabled(turtle_handle h)
+{
+    (void)h;
+    return 0;
+}
+
+int turtle_get_tuning(turtle_handle h, turtle_tuning *t)
+{
+    if (!h || !t) return fail(TURTLE_ERR_INVALID, "NULL argument");
+    *t = h->tune;
+    return TURTLE_OK;
+}
+
+int turtle_set_tuning(turtle_handle h, const turtle_tuning *t)
+{
+    if (!h || !t) return fail(TURTLE_ERR_INVALID, "NULL argument");
+    if (!valid_tuning(*t, h->kb)) return fail(TURTLE_ERR_INVALID, "tuning out of range");
+    h->tune = *t;
+    return TURTLE_OK;
+}
+
+int turtle_sort_device(turtle_handle h, const void *keys_in, const int32_t *idx_in, void *keys_out, int32_t *idx_out,
+                       int64_t n, void *stream)
+{
+    if (!h) return fail(TURTLE_ERR_INVALID, "NULL handle");
+    if (n < 0) return fail(TURTLE_ERR_INVALID, "negative n");
+    if (n > h->max_n) return fail(TURTLE_ERR_CAPACITY, "n exceeds the handle's max_n");
+    if (n > 0 && (!keys_in || !keys_out)) return fail(TURTLE_ERR_INVALID, "NULL key buffer");
+    if ((idx_in == nullptr) != (idx_out == nullptr)) return fail(TURTLE_ERR_INVALID, "idx_in and idx_out must both be set");
+    if (idx_in && !h->track) return fail(TURTLE_ERR_INVALID, "handle created without index tracking");
+    return run_engine(h, keys_in, (const uint32_t *)idx_in, keys_out, (uint32_t *)idx_out, (uint64_t)n, nullptr, 1,
+                      stream_of(h, stream));
+}
+
+int turtle_segmented_sort_device(turtle_handle h, const void *keys_in, const int32_t *idx_in, void *keys_out,
+                                 int32_t *idx_out, const int64_t *offsets, int64_t num_segments, void *stream)
+{
+    if (!h) return fail(TURTLE_ERR_INVALID, "NULL handle");
+    if (num_segments < 0 || (num_segments > 0 && !offsets)) return fail(TURTLE_ERR_INVALID, "bad segment list");
+    if ((idx_in == nullptr) != (idx_out == nullptr)) return fail(TURTLE_ERR_INVALID, "idx_in and idx_out must both be set");
+    if (idx_in && !h->track) return fail(TURTLE_ERR_INVALID, "handle created without index tracking");
+    if (num_segments >= (int64_t)1 << 30) return fail(TURTLE_ERR_CAPACITY, "too many segments");
+    if (num_segments == 0) return TURTLE_OK;
+    if (offsets[0] < 0) return fail(TURTLE_ERR_INVALID, "negative offset");
+    for (int64_t s = 0; s < num_segments; s++)
+        if (offsets[s + 1] < offsets[s]) return fail(TURTLE_ERR_INVALID, "offsets must be non-decreasing");
+    const int64_t total = offsets[num_segments];
+    if (total > h->max_n) return fail(TURTLE_ERR_CAPACITY, "segments exceed the handle's max_n");
+    if (total > 0 && (!keys_in || !keys_out)) return fail(TURTLE_ERR_INVALID, "NULL key buffer");
+    hipStream_t st = stream_of(h, stream);
+    // the previous run's offsets may still be in use by the device
+    HIPCHK(hipStreamSynchronize(h->last_stream), "hipStreamSynchronize");
+    if (st != h->last_stream) HIPCHK(hipStreamSynchronize(st), "hipStreamSynchronize");
+    const size_t need = (size_t)num_segments + 1;
+    if (h->dev_off_cap < need) {
+        if (h->dev_off) (void)hipFree(h->dev_off);
+        h->dev_off = nullptr;
+        h->dev_off_cap = 0;
+        HIPCHK(hipMalloc((void **)&h->dev_off, need * sizeof(uint32_t)), "hipMalloc(offsets)");
+        h->dev_off_cap = need;
+    }
+    h->host_off.resize(need);
+    for (size_t s = 0; s < need; s++) h->host_off[s] = (uint32_t)offsets[s];
+    HIPCHK(hipMemcpyAsync(h->dev_off, h->host_off.data(), need * sizeof(uint32_t), hipMemcpyHostToDevice, st),
+           "hipMemcpyAsync(offsets)");
+    return run_engine(h, keys_in, (const uint32_t *)idx_in, keys_out, (uint32_t *)idx_out, (uint64_t)total,
+                      h->dev_off, (uint32_t)num_segments, st);
+}
+
+int turtle_multisort_device(turtle_handle h, int dtype, int64_t arr_size, const void *keys_in, void *keys_out,
+                            int64_t num_arrays, void *stream)
+{
+    if (!h) return fail(TURTLE_ERR_INVALID, "NULL handle");
+    if (dtype != h->dtype) return fail(TURTLE_ERR_DTYPE, "dtype differs from the handle's");
+    if (arr_size < 0 || num_arrays < 0) return fail(TURTLE_ERR_INVALID, "negative size");
+    if (arr_size == 0 || num_arrays == 0) return TURTLE_OK;
+    if (!keys_in || !keys_out) return fail(TURTLE_ERR_INVALID, "NULL key buffer");
+    if (arr_size >= (int64_t)1 << 31 || num_arrays > ((int64_t)1 << 62) / arr_size)
+        return fail(TURTLE_ERR_CAPACITY, "batch too large");
+    hipStream_t st = stream_of(h, stream);
+    if ((uint64_t)arr_size <= ts::multisort_max_size(h->kb)) {
+        HIPCHK(hipEventRecord(h->ev0, st), "hipEventRecord");
+        HIPCHK(ts::multisort_run(h->kb, h->flip, (uint32_t)arr_size, keys_in, keys_out, (uint64_t)num_arrays, st),
+               "multisort launch");
+        HIPCHK(hipEventRecord(h->ev1, st), "hipEventRecord");
+        h->timed = true;
+        h->last_stream = st;
+        h->last_n = (uint64_t)arr_size * num_arrays;
+        h->last_engine = false;
+        return TURTLE_OK;
+    }
+    // longer arrays: equal segments through the persistent engine
+    const int64_t total = arr_size * num_arrays;
+    if (total > h->max_n) return fail(TURTLE_ERR_CAPACITY, "arrays exceed the handle's max_n");
+    std::vector<int64_t> off((size_t)num_arrays + 1);
+    for (int64_t a = 0; a <= num_arrays; a++) off[a] = a * arr_size;
+    return turtle_segmented_sort_device(h, keys_in, nullptr, keys_out, nullptr, off.data(), num_arrays, stream);
+}
+
+int turtle_synchronize(turtle_handle h, turtle_stats *st)
+{
+    if (!h) return fail(TURTLE_ERR_INVALID, "NULL handle");
+    HIPCHK(hipStreamSynchronize(h->last_stream), "hipStreamSynchronize");
+    uint32_t err = 0;
+    int rc = read_stats(h, st, &err);
+    if (rc) return rc;
+    rc = error_status(err);
+    if (rc == TURTLE_ERR_QUEUE_OVERFLOW) {
+        // grow the tables so that the next run fits
+        (void)ensure_tables(h, h->last_total, h->last_nroots, 4);
+    }
+    return rc;
+}
+
+int turtle_sort_start(turtle_handle h, void *keys, int32_t *idx, int64_t n)
+{
+    if (!h) return fail(TURTLE_ERR_INVALID, "NULL handle");
+    if (n < 0) return fail(TURTLE_ERR_INVALID, "negative n");
+    if (n > h->max_n) return fail(TURTLE_ERR_CAPACITY, "n exceeds the handle's max_n");
+    if (n > 0 && !keys) return fail(TURTLE_ERR_INVALID, "NULL keys");
+    if (idx && !h->track) return fail(TURTLE_ERR_INVALID, "handle created without index tracking");
+    if (h->started) return fail(TURTLE_ERR_STATE, "StartSorting called twice without Sync");
+    h->t_start = now_s();
+    h->host_keys = keys;
+    h->host_idx = idx;
+    h->host_n = n;
+    if (n > 0) {
+        HIPCHK(hipMemcpyAsync(h->kin, keys, (size_t)n * h->kb, hipMemcpyHostToDevice, h->stream), "H2D keys");
+        if (idx) HIPCHK(hipMemcpyAsync(h->iin, idx, (size_t)n * 4, hipMemcpyHostToDevice, h->stream), "H2D idx");
+    }
+    int rc = run_engine(h, h->kin, idx ? h->iin : nullptr, h->kout, idx ? h->iout : nullptr, (uint64_t)n, nullptr, 1,
+                        h->stream);
+    if (rc) return rc;
+    h->started = true;
+    return TURTLE_OK;
+}
+
+int turtle_sort_sync(turtle_handle h, double *seconds)
+{
+    if (!h) return fail(TURTLE_ERR_INVALID, "NULL handle");
+    if (!h->started) return fail(TURTLE_ERR_STATE, "Sync without StartSorting");
+    h->started = false;
+    int rc = turtle_synchronize(h, nullptr);
+    if (rc == TURTLE_ERR_QUEUE_OVERFLOW) {
+        // the input is still intact in the device input buffer: run again
+        // with the grown tables
+        const bool idx = h->host_idx != nullptr;
+        rc = run_engine(h, h->kin, idx ? h->iin : nullptr, h->kout, idx ? h->iout : nullptr, (uint64_t)h->host_n,
+                        nullptr, 1, h->stream);
+        if (!rc) rc = turtle_synchronize(h, nullptr);
+    }
+    if (rc) return rc;
+    if (h->host_n > 0) {
+        HIPCHK(hipMemcpy(h->host_keys, h->kout, (size_t)h->host_n * h->kb, hipMemcpyDeviceToHost), "D2H keys");
+        if (h->host_idx) HIPCHK(hipMemcpy(h->host_idx, h->iout, (size_t)h->host_n * 4, hipMemcpyDeviceToHost), "D2H idx");
+    }
+    if (seconds) *seconds = now_s() - h->t_start;
+    h->host_keys = nullptr;
+    h->host_idx = nullptr;
+    return TURTLE_OK;
+}
+
+int turtle_multisort(turtle_handle h, int dtype, int64_t arr_size, int block_size, int use_lds, int64_t num_arrays,
+                     void *host_data, double *seconds)
+{
+    (void)block_size;
+    (void)use_lds;
+    if (!h) return fail(TURTLE_ERR_INVALID, "NULL handle");
+    const double t0 = now_s();
+    if (dtype != h->dtype) return fail(TURTLE_ERR_DTYPE, "dtype differs from the handle's");
+    if (arr_size < 0 || num_arrays < 0) return fail(TURTLE_ERR_INVALID, "negative size");
+    if (arr_size > 0 && num_arrays > h->max_n / (arr_size ? arr_size : 1))
+        return fail(TURTLE_ERR_CAPACITY, "numArrays*arrSize exceeds the handle's max_n");
+    const int64_t total = arr_size * num_arrays;
+    if (total > 0) {
+        if (!host_data) return fail(TURTLE_ERR_INVALID, "NULL host data");
+        HIPCHK(hipMemcpyAsync(h->kin, host_data, (size_t)total * h->kb, hipMemcpyHostToDevice, h->stream), "H2D");
+        int rc = turtle_multisort_device(h, dtype, arr_size, h->kin, h->kout, num_arrays, h->stream);
+        if (rc) return rc;
+        rc = turtle_synchronize(h, nullptr);
+        if (rc) return rc;
+        HIPCHK(hipMemcpy(host_data, h->kout, (size_t)total * h->kb, hipMemcpyDeviceToHost), "D2H");
+    }
+    if (seconds) *seconds = now_s() - t0;
+    return TURTLE_OK;
+}
+
+int turtle_fill_keys(int dtype, void *keys, int64_t n, uint64_t seed, int shift, void *stream)
+{
+    const int kb = dtype_bytes(dtype);
+    if (!kb) return fail(TURTLE_ERR_DTYPE, "unknown dtype");
+    if (n < 0 || shift < 0 || shift > 63 || (n > 0 && !keys)) return fail(TURTLE_ERR_INVALID, "bad argument");
+    HIPCHK(ts::fill_keys(kb, keys, (uint64_t)n, seed, shift, (hipStream_t)stream), "fill_keys");
+    return TURTLE_OK;
+}
+
+int turtle_fill_iota(int32_t *idx, int64_t n, void *stream)
+{
+    if (n < 0 || (n > 0 && !idx)) return fail(TURTLE_ERR_INVALID, "bad argument");
+    HIPCHK(ts::fill_iota((uint32_t *)idx, (uint64_t)n, (hipStream_t)stream), "fill_iota");
+    return TURTLE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,253 @@
+// ts_device.h -- device-side building blocks shared by the gfx950 kernels:
+// key traits, wave64 helpers, agent-scope synchronisation, block scans and
+// the LDS-resident stable block sort.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ts {
+
+constexpr int WAVE = 64;
+
+// ---- key traits ------------------------------------------------------------
+// Keys are moved in their stored representation.  Comparisons use an
+// unsigned "order" value: signed keys have their sign bit flipped, which maps
+// the signed order onto the unsigned order of the same width.
+template <int KB> struct KeyT;
+template <> struct KeyT<1> { typedef uint8_t U; };
+template <> struct KeyT<2> { typedef uint16_t U; };
+template <> struct KeyT<4> { typedef uint32_t U; };
+template <> struct KeyT<8> { typedef uint64_t U; };
+
+template <typename U>
+__device__ __forceinline__ U key_max() { return (U)~(U)0; }
+
+// ---- wave helpers ----------------------------------------------------------
+__device__ __forceinline__ uint32_t lane() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
+// number of set bits of m strictly below this lane
+__device__ __forceinline__ uint32_t mask_rank(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
+{
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        uint32_t o = __shfl_up(v, d, WAVE);
+        if ((int)lane() >= d) v += o;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v)
+{
+#pragma unroll
+    for (int d = WAVE / 2; d >= 1; d >>= 1) v += __shfl_xor(v, d, WAVE);
+    return v;
+}
+
+// Lanes whose value v (< 2^BITS, or invalid) equals this lane's: AND of the
+// per-bit ballots, restricted to valid lanes.
+template <int BITS>
+__device__ __forceinline__ uint64_t match_peers(uint32_t v, bool valid)
+{
+    uint64_t m = ballot(valid);
+#pragma unroll
+    for (int b = 0; b < BITS; b++) {
+        const uint64_t bb = ballot((v >> b) & 1u);
+        m &= ((v >> b) & 1u) ? bb : ~bb;
+    }
+    return m;
+}
+
+// ---- agent-scope synchronisation ----------------------------------------------
+// Work handed between workgroups (possibly on different XCDs, each with its
+// own L2) is published with a release fence before the signalling atomic and
+// consumed with an acquire fence after it.
+__device__ __forceinline__ void fence_release() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
+__device__ __forceinline__ void fence_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+
+__device__ __forceinline__ uint32_t atomic_add(uint32_t *p, uint32_t v)
+{
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long atomic_add64(unsigned long long *p, unsigned long long v)
+{
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int32_t atomic_add_i(int32_t *p, int32_t v)
+{
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t atomic_or(uint32_t *p, uint32_t v)
+{
+    return __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t atomic_max(uint32_t *p, uint32_t v)
+{
+    return __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t load_acquire(const uint32_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t load_relaxed(const uint32_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_relaxed(uint32_t *p, uint32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---- block-wide exclusive scan (NT threads) -------------------------------------
+// red: LDS scratch of at least NT/64 + 1 words.  Returns the exclusive prefix
+// of v over thread order; *total receives the block sum.  Contains barriers:
+// every thread of the block must call it.
+template <int NT>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *red, uint32_t *total)
+{
+    constexpr int NW = NT / WAVE;
+    const int w = threadIdx.x / WAVE;
+    uint32_t inc = wave_incl_scan(v);
+    if (lane() == WAVE - 1) red[w] = inc;
+    __syncthreads();
+    uint32_t wpre = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < NW; i++) {
+        uint32_t s = red[i];
+        wpre += (i < w) ? s : 0u;
+        tot += s;
+    }
+    __syncthreads();
+    *total = tot;
+    return wpre + inc - v;
+}
+
+// ---- LDS layout with one padding slot every ITEMS elements -------------------
+// Thread t of a blocked arrangement touches elements [t*ITEMS, (t+1)*ITEMS);
+// the padding makes the stride ITEMS+1 (odd), so lanes hit distinct banks.
+template <int ITEMS>
+__device__ __forceinline__ uint32_t pidx(uint32_t i) { return i + i / ITEMS; }
+
+template <int ITEMS>
+__host__ __device__ constexpr uint32_t padded_len(uint32_t n) { return n + n / ITEMS + 1; }
+
+// ---- stable LDS block sort ---------------------------------------------------------
+// Sorts (order-value, payload) pairs held in LDS: sk[] keys in order
+// representation, si[] 32-bit payloads (ignored when !IDX), both in the pidx
+// layout.  n_eff is a power of two, ITEMS <= n_eff <= NT*ITEMS, and elements
+// [n, n_eff) have been filled with key_max (they stay behind every real
+// element because every step is stable).  Threads >= n_eff/ITEMS only take
+// part in the barriers.
+// Step 1: each thread sorts its ITEMS consecutive elements in registers with
+//         odd-even transposition (stable: swaps only strictly greater pairs);
+// Step 2: log2(n_eff/ITEMS) rounds of stable merge-path merges.
+// The result is left in LDS in the same layout.
+template <typename U, bool IDX, int NT, int ITEMS>
+__device__ void block_sort(U *sk, uint32_t *si, uint32_t n_eff)
+{
+    const uint32_t t = threadIdx.x;
+    const bool active = t * ITEMS < n_eff;
+    U k[ITEMS];
+    uint32_t v[ITEMS];
+    if (active) {
+#pragma unroll
+        for (int i = 0; i < ITEMS; i++) {
+            k[i] = sk[pidx<ITEMS>(t * ITEMS + i)];
+            if (IDX) v[i] = si[pidx<ITEMS>(t * ITEMS + i)];
+        }
+#pragma unroll
+        for (int r = 0; r < ITEMS; r++) {
+#pragma unroll
+            for (int i = (r & 1); i + 1 < ITEMS; i += 2) {
+                bool sw = k[i + 1] < k[i];
+                U a = k[i], b = k[i + 1];
+                k[i] = sw ? b : a;
+                k[i + 1] = sw ? a : b;
+                if (IDX) {
+                    uint32_t x = v[i], y = v[i + 1];
+                    v[i] = sw ? y : x;
+                    v[i + 1] = sw ? x : y;
+                }
+            }
+        }
+    }
+    for (uint32_t run = ITEMS; run < n_eff; run *= 2) {
+        __syncthreads();
+        if (active) {
+#pragma unroll
+            for (int i = 0; i < ITEMS; i++) {
+                sk[pidx<ITEMS>(t * ITEMS + i)] = k[i];
+                if (IDX) si[pidx<ITEMS>(t * ITEMS + i)] = v[i];
+            }
+        }
+        __syncthreads();
+        if (active) {
+            const uint32_t pos = t * ITEMS;
+            const uint32_t base = pos & ~(2 * run - 1);
+            const uint32_t diag = pos - base;
+            const uint32_t A = base, B = base + run;
+            // merge path: number of A elements among the first diag outputs
+            uint32_t lo = diag > run ? diag - run : 0;
+            uint32_t hi = diag < run ? diag : run;
+            while (lo < hi) {
+                uint32_t mid = (lo + hi) >> 1;
+                U bk = sk[pidx<ITEMS>(B + diag - 1 - mid)];
+                U ak = sk[pidx<ITEMS>(A + mid)];
+                if (!(bk < ak)) lo = mid + 1; else hi = mid;
+            }
+            // serial stable merge of ITEMS outputs; ties take A.  One LDS
+            // load per step (the successor of the element taken); source
+            // positions are kept so the payloads are gathered afterwards.
+            // Positions A+run and B+run stay inside the padded array.
+            uint32_t ai = A + lo, bi = B + (diag - lo);
+            const uint32_t aend = B, bend = B + run;
+            U ka = sk[pidx<ITEMS>(ai)];
+            U kb = sk[pidx<ITEMS>(bi)];
+            uint32_t src[ITEMS];
+#pragma unroll
+            for (int i = 0; i < ITEMS; i++) {
+                const bool takeA = (bi >= bend) || (ai < aend && !(kb < ka));
+                k[i] = takeA ? ka : kb;
+                src[i] = takeA ? ai : bi;
+                ai += takeA ? 1u : 0u;
+                bi += takeA ? 0u : 1u;
+                const U nk = sk[pidx<ITEMS>(takeA ? ai : bi)];
+                ka = takeA ? nk : ka;
+                kb = takeA ? kb : nk;
+            }
+            if (IDX) {
+#pragma unroll
+                for (int i = 0; i < ITEMS; i++) v[i] = si[pidx<ITEMS>(src[i])];
+            }
+        }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+        for (int i = 0; i < ITEMS; i++) {
+            sk[pidx<ITEMS>(t * ITEMS + i)] = k[i];
+            if (IDX) si[pidx<ITEMS>(t * ITEMS + i)] = v[i];
+        }
+    }
+    __syncthreads();
+}
+
+__host__ __device__ constexpr uint32_t next_pow2(uint32_t x)
+{
+    uint32_t p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+__device__ __forceinline__ uint32_t next_pow2_dev(uint32_t x)
+{
+    return x <= 1 ? 1u : 1u << (32 - __clz(x - 1));
+}
+
+}  // namespace ts

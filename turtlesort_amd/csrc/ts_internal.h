@@ -1,0 +1,30 @@
+// ts_internal.h -- declarations shared between the kernel files and the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "ts_engine.h"
+
+namespace ts {
+
+// persistent engine (ts_engine.hip)
+int engine_occupancy(int kb, bool idx, int splitters);
+bool engine_supports(int splitters);   // splitter counts compiled in
+uint32_t engine_tile(int kb);      // elements per partition tile
+uint32_t engine_cap(int kb);       // largest segment the LDS sort takes
+// Launches reset + roots + drain on st.  dev_off: nseg+1 segment offsets
+// (uint32) in device memory, or nullptr for the single segment
+// [0, single_len) (nseg must then be 1).
+hipError_t engine_run(const Params &p, int kb, bool idx, int splitters, int grid,
+                      const uint32_t *dev_off, uint32_t nseg, uint32_t single_len, hipStream_t st);
+
+// MultiSort kernels (ts_multisort.hip): arrays of arr_size keys, in -> out.
+// Returns hipErrorInvalidValue when arr_size exceeds multisort_max_size(kb).
+hipError_t multisort_run(int kb, uint64_t flip, uint32_t arr_size, const void *in, void *out,
+                         uint64_t num_arrays, hipStream_t st);
+uint32_t multisort_max_size(int kb);
+
+// synthetic inputs (ts_util.hip)
+hipError_t fill_keys(int kb, void *keys, uint64_t n, uint64_t seed, int shift, hipStream_t st);
+hipError_t fill_iota(uint32_t *idx, uint64_t n, hipStream_t st);
+
+}  // namespace ts

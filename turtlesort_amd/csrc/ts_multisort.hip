@@ -1,0 +1,79 @@
+// ts_multisort.hip -- batched equal-size sort (MultiSort) for gfx950.
+//
+// The reference gives each CUDA thread one array and heap-sorts it in an
+// interleaved layout (turtle-multi-sort.cuh:86-182); at 1024 keys per array
+// that is ~20K dependent compare/swaps per thread.  Here one workgroup owns
+// one array: a coalesced load into LDS, the stable block sort of
+// ts_device.h (register odd-even transposition + LDS merge path), and a
+// coalesced store.  Every array is sorted, including a remainder that is not
+// a multiple of the reference's BlockSize (turtle-multi-sort.cuh:200-202
+// skips it), and the input buffer is the one that is sorted
+// (turtle-multi-sort.cuh:152 sorts an unfilled scratch buffer).
+#include "ts_device.h"
+#include "ts_internal.h"
+
+namespace ts {
+
+template <typename U, int NT, int ITEMS>
+__global__ void __launch_bounds__(NT) ms_kernel(const U *__restrict__ in, U *__restrict__ out, uint32_t arr_size,
+                                                 U flip)
+{
+    constexpr uint32_t CAPA = (uint32_t)NT * ITEMS;
+    __shared__ U sk[padded_len<ITEMS>(CAPA)];
+    const size_t base = (size_t)blockIdx.x * arr_size;
+    uint32_t n_eff = next_pow2_dev(arr_size);
+    if (n_eff < (uint32_t)ITEMS) n_eff = ITEMS;
+    for (uint32_t i = threadIdx.x; i < n_eff; i += NT)
+        sk[pidx<ITEMS>(i)] = i < arr_size ? (U)(in[base + i] ^ flip) : key_max<U>();
+    __syncthreads();
+    block_sort<U, false, NT, ITEMS>(sk, nullptr, n_eff);
+    for (uint32_t i = threadIdx.x; i < arr_size; i += NT) out[base + i] = (U)(sk[pidx<ITEMS>(i)] ^ flip);
+}
+
+template <typename U>
+static hipError_t ms_launch(uint64_t flip, uint32_t arr_size, const void *in, void *out, uint64_t num_arrays,
+                            hipStream_t st)
+{
+    const U f = (U)flip;
+    const U *i = (const U *)in;
+    U *o = (U *)out;
+    uint32_t ne = next_pow2(arr_size);
+    // grid.x is limited to 2^31-1 workgroups; larger batches go in chunks
+    const uint64_t chunk = 1ull << 30;
+    for (uint64_t a0 = 0; a0 < num_arrays; a0 += chunk) {
+        const uint64_t na = (num_arrays - a0) < chunk ? (num_arrays - a0) : chunk;
+        const U *ii = i + a0 * arr_size;
+        U *oo = o + a0 * arr_size;
+        if (ne <= 256)
+            hipLaunchKernelGGL((ms_kernel<U, 64, 4>), dim3((uint32_t)na), dim3(64), 0, st, ii, oo, arr_size, f);
+        else if (ne <= 1024)
+            hipLaunchKernelGGL((ms_kernel<U, 128, 8>), dim3((uint32_t)na), dim3(128), 0, st, ii, oo, arr_size, f);
+        else if (ne <= 4096)
+            hipLaunchKernelGGL((ms_kernel<U, 256, 16>), dim3((uint32_t)na), dim3(256), 0, st, ii, oo, arr_size, f);
+        else if (ne <= 8192)
+            hipLaunchKernelGGL((ms_kernel<U, 512, 16>), dim3((uint32_t)na), dim3(512), 0, st, ii, oo, arr_size, f);
+        else
+            return hipErrorInvalidValue;
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+uint32_t multisort_max_size(int) { return 8192; }
+
+hipError_t multisort_run(int kb, uint64_t flip, uint32_t arr_size, const void *in, void *out, uint64_t num_arrays,
+                         hipStream_t st)
+{
+    if (arr_size == 0 || num_arrays == 0) return hipSuccess;
+    if (arr_size > multisort_max_size(kb)) return hipErrorInvalidValue;
+    switch (kb) {
+    case 1: return ms_launch<uint8_t>(flip, arr_size, in, out, num_arrays, st);
+    case 2: return ms_launch<uint16_t>(flip, arr_size, in, out, num_arrays, st);
+    case 4: return ms_launch<uint32_t>(flip, arr_size, in, out, num_arrays, st);
+    case 8: return ms_launch<uint64_t>(flip, arr_size, in, out, num_arrays, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace ts
