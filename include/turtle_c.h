@@ -134,6 +134,19 @@ int turtle_segmented_sort_device(turtle_handle h, const void *keys_in, const int
 int turtle_multisort_device(turtle_handle h, int dtype, int64_t arr_size,
                             const void *keys_in, void *keys_out, int64_t num_arrays, void *stream);
 
+/* hipGraph form of turtle_segmented_sort_device: the reset, root and
+ * persistent-drain launches are captured once (offsets copied to a device
+ * buffer owned by the graph; no memset nodes) and replayed by
+ * turtle_graph_launch on any stream.  The graph uses the handle's engine
+ * tables, so at most one run per handle may be in flight.  keys_out must
+ * differ from keys_in.  No reference counterpart. */
+typedef struct turtle_graph *turtle_graph_handle;
+int turtle_segmented_graph_create(turtle_handle h, const void *keys_in, const int32_t *idx_in,
+                                  void *keys_out, int32_t *idx_out, const int64_t *offsets,
+                                  int64_t num_segments, turtle_graph_handle *out);
+int turtle_graph_launch(turtle_graph_handle g, void *stream);
+int turtle_graph_destroy(turtle_graph_handle g);
+
 /* Waits for the handle's last device work and fills *st (may be NULL). */
 int turtle_synchronize(turtle_handle h, turtle_stats *st);
 

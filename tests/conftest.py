@@ -25,3 +25,19 @@ def torch_dev():
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
     return torch
+
+
+def h2d(torch, a):
+    """numpy -> device through a freshly allocated pinned host buffer (the
+    HIP runtime is never handed pageable memory)"""
+    import numpy as np
+    t = torch.from_numpy(np.ascontiguousarray(a)).pin_memory()
+    return t.to("cuda")
+
+
+def d2h(torch, t):
+    """device -> numpy through a pinned host buffer"""
+    out = torch.empty(tuple(t.shape), dtype=t.dtype, pin_memory=True)
+    out.copy_(t)
+    torch.cuda.synchronize()
+    return out.numpy()

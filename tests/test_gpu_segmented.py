@@ -1,0 +1,70 @@
+"""Batched uneven segments in one persistent launch: each segment equal to
+the oracle's sort of it (keys bit-exact, stable index)."""
+import numpy as np
+import pytest
+
+from turtlesort_amd import TurtleSort, workloads as wl
+
+from conftest import d2h, h2d
+
+pytestmark = pytest.mark.gpu
+
+
+def check_segments(orc, keys, offsets, ok, oi):
+    ek, _ = orc.segmented_sort(keys, np.arange(keys.size, dtype=np.int32), offsets)
+    np.testing.assert_array_equal(ok, ek)
+    for s in range(len(offsets) - 1):
+        a, b = offsets[s], offsets[s + 1]
+        if b > a:
+            np.testing.assert_array_equal(oi[a:b], a + np.argsort(keys[a:b], kind="stable"))
+
+
+@pytest.mark.parametrize("dtype", [np.int64, np.int32, np.uint16, np.int8])
+def test_segmented_small(orc, torch_dev, dtype):
+    torch = torch_dev
+    lens = wl.loguniform_lengths(17, 400, 1, 40000)
+    lens[::7] = 0                               # empty segments too
+    off = wl.segment_offsets(lens)
+    n = int(off[-1])
+    keys = wl.gen_keys(dtype, n, 23, 0)
+    sorter = TurtleSort(dtype, n)
+    tdt = {np.dtype(np.int64): torch.int64, np.dtype(np.int32): torch.int32,
+           np.dtype(np.int8): torch.int8, np.dtype(np.uint16): torch.int16}[np.dtype(dtype)]
+    kin = h2d(torch, keys.view(np.int16) if dtype == np.uint16 else keys)
+    iin = torch.arange(n, dtype=torch.int32, device="cuda")
+    kout = torch.empty_like(kin)
+    iout = torch.empty_like(iin)
+    sorter.segmented_sort_device(kin, kout, off, iin, iout)
+    sorter.synchronize()
+    ok = d2h(torch, kout)
+    if dtype == np.uint16:
+        ok = ok.view(np.uint16)
+    check_segments(orc, keys, off, ok, d2h(torch, iout))
+    sorter.close()
+
+
+def test_segmented_config4_shape(torch_dev):
+    """20,000 segments log-uniform in [16, 262144], int64 keys + int32 idx:
+    per-segment sortedness, stability and permutation checked on the GPU."""
+    torch = torch_dev
+    lens = wl.loguniform_lengths(wl.SEED_C4_LENGTHS, wl.C4_SEGMENTS, wl.C4_LO, wl.C4_HI)
+    off = wl.segment_offsets(lens)
+    n = int(off[-1])
+    sorter = TurtleSort(np.int64, n)
+    from turtlesort_amd import fill_keys
+    kin = torch.empty(n, dtype=torch.int64, device="cuda")
+    fill_keys(kin, wl.SEED_C4_KEYS, wl.SHIFT_63BIT)
+    iin = torch.arange(n, dtype=torch.int32, device="cuda")
+    kout = torch.empty_like(kin)
+    iout = torch.empty_like(iin)
+    sorter.segmented_sort_device(kin, kout, off, iin, iout)
+    st = sorter.synchronize()
+    seg_id = torch.repeat_interleave(torch.arange(len(lens), device="cuda"), h2d(torch, lens))
+    same = seg_id[1:] == seg_id[:-1]
+    assert bool((~same | (kout[1:] >= kout[:-1])).all())
+    assert bool((~same | (kout[1:] != kout[:-1]) | (iout[1:] > iout[:-1])).all())
+    assert torch.equal(kin[iout.long()], kout)
+    assert torch.equal(seg_id[iout.long()], seg_id)        # indices stay in their segment
+    assert bool((torch.bincount(iout.long(), minlength=n) == 1).all())
+    assert st["error"] == 0
+    sorter.close()

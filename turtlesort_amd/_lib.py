@@ -82,6 +82,9 @@ SIGNATURES = {
     "turtle_segmented_sort_device": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp]),
     "turtle_multisort_device": (_i32, [_vp, _i32, _i64, _vp, _vp, _i64, _vp]),
     "turtle_synchronize": (_i32, [_vp, ctypes.POINTER(Stats)]),
+    "turtle_segmented_graph_create": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, ctypes.POINTER(_vp)]),
+    "turtle_graph_launch": (_i32, [_vp, _vp]),
+    "turtle_graph_destroy": (_i32, [_vp]),
     "turtle_get_tuning": (_i32, [_vp, ctypes.POINTER(Tuning)]),
     "turtle_set_tuning": (_i32, [_vp, ctypes.POINTER(Tuning)]),
     "turtle_fill_keys": (_i32, [_i32, _vp, _i64, _u64, _i32, _vp]),

@@ -89,6 +89,7 @@ struct turtle_sorter {
     uint32_t *iin = nullptr, *iout = nullptr, *it0 = nullptr, *it1 = nullptr;
     // engine tables
     ts::Ctrl *ctrl = nullptr;
+    ts::Ctrl *ctrl_host = nullptr;   // pinned readback copy
     ts::Slot *q = nullptr;
     ts::Seg *segs = nullptr;
     uint32_t *tiles = nullptr;
@@ -106,6 +107,11 @@ struct turtle_sorter {
     int64_t host_n = 0;
     bool started = false;
     double t_start = 0.0;
+    // pinned staging for host <-> device copies: user memory is only ever
+    // touched by host memcpy, never handed to the HIP runtime
+    void *stage[2] = {nullptr, nullptr};
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    bool stage_used[2] = {false, false};
     // what the last engine run needs to be run again after its tables grow
     bool last_engine = false;
     uint64_t last_total = 0;
@@ -188,6 +194,7 @@ ts::Params make_params(turtle_sorter *h, const void *kin, const uint32_t *iin, v
     p.tilecap = h->tilecap;
     p.cutoff = (uint32_t)h->tune.lds_cutoff;
     p.tiles_per_task = (uint32_t)(h->tune.tiles_per_task > 0 ? h->tune.tiles_per_task : 1);
+    p.n_total = 0;   // set by run_engine
     p.flip = h->flip;
     const uint64_t ms = env_u64("TURTLE_WATCHDOG_MS", 20000);
     p.timeout_ticks = ms * 100000ull;   // wall_clock64 runs at 100 MHz
@@ -202,7 +209,7 @@ hipStream_t stream_of(turtle_sorter *h, void *stream)
 // One engine run over device buffers.  In-place requests (kin == kout) first
 // copy the input into the second ping-pong buffer, which depth 0 only reads.
 int run_engine(turtle_sorter *h, const void *kin, const uint32_t *iin, void *kout, uint32_t *iout, uint64_t total,
-               const uint32_t *dev_off, uint32_t nseg, hipStream_t st)
+               const uint32_t *dev_off, uint32_t nseg, hipStream_t st, bool capturing = false)
 {
     const bool idx = iin != nullptr && iout != nullptr;
     int rc = ensure_tables(h, total, nseg, 1);
@@ -216,11 +223,12 @@ int run_engine(turtle_sorter *h, const void *kin, const uint32_t *iin, void *kou
         iin = h->it1;
     }
     ts::Params p = make_params(h, kin, idx ? iin : nullptr, kout, idx ? iout : nullptr);
-    HIPCHK(hipEventRecord(h->ev0, st), "hipEventRecord");
+    p.n_total = (uint32_t)total;
+    if (!capturing) HIPCHK(hipEventRecord(h->ev0, st), "hipEventRecord");
     HIPCHK(ts::engine_run(p, h->kb, idx, h->tune.splitters, grid_of(h, idx), dev_off, nseg, (uint32_t)total, st),
            "engine launch");
-    HIPCHK(hipEventRecord(h->ev1, st), "hipEventRecord");
-    h->timed = true;
+    if (!capturing) HIPCHK(hipEventRecord(h->ev1, st), "hipEventRecord");
+    h->timed = !capturing;
     h->last_stream = st;
     h->last_n = total;
     h->last_engine = true;
@@ -231,8 +239,10 @@ int run_engine(turtle_sorter *h, const void *kin, const uint32_t *iin, void *kou
 
 int read_stats(turtle_sorter *h, turtle_stats *st, uint32_t *err_out)
 {
-    ts::Ctrl c;
-    HIPCHK(hipMemcpy(&c, h->ctrl, sizeof(c), hipMemcpyDeviceToHost), "hipMemcpy(ctrl)");
+    if (!h->ctrl_host) HIPCHK(hipHostMalloc((void **)&h->ctrl_host, sizeof(ts::Ctrl), hipHostMallocDefault), "hipHostMalloc");
+    HIPCHK(hipMemcpyAsync(h->ctrl_host, h->ctrl, sizeof(ts::Ctrl), hipMemcpyDeviceToHost, h->last_stream), "D2H(ctrl)");
+    HIPCHK(hipStreamSynchronize(h->last_stream), "hipStreamSynchronize");
+    const ts::Ctrl c = *h->ctrl_host;
     float ms = 0.f;
     if (h->timed) HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1), "hipEventElapsedTime");
     if (st) {
@@ -260,6 +270,8 @@ int error_status(uint32_t err)
 {
     if (err == 0) return TURTLE_OK;
     if (err & ts::ERR_TIMEOUT) return fail(TURTLE_ERR_TIMEOUT, "persistent sort kernel watchdog expired");
+    if (err & ts::ERR_RANGE)
+        return fail(TURTLE_ERR_DEVICE, "device range check failed (access skipped)");
     if (err & (ts::ERR_QUEUE | ts::ERR_SEGS | ts::ERR_TILES))
         return fail(TURTLE_ERR_QUEUE_OVERFLOW, "device work queue or segment/tile table overflow");
     return fail(TURTLE_ERR_DEVICE, "inconsistent partition detected on the device");
@@ -282,6 +294,63 @@ turtle_tuning default_tuning(int kb)
     t.tiles_per_task = 1;
     t.workgroups_per_cu = 0;
     return t;
+}
+
+constexpr size_t STAGE_BYTES = 16u << 20;
+
+int ensure_staging(turtle_sorter *h)
+{
+    for (int k = 0; k < 2; k++) {
+        if (!h->stage[k]) HIPCHK(hipHostMalloc(&h->stage[k], STAGE_BYTES, hipHostMallocDefault), "hipHostMalloc(staging)");
+        if (!h->stage_ev[k]) HIPCHK(hipEventCreateWithFlags(&h->stage_ev[k], hipEventDisableTiming), "hipEventCreate");
+    }
+    return TURTLE_OK;
+}
+
+// host -> device through the two pinned chunks: the memcpy into one chunk
+// overlaps the DMA out of the other
+int stage_h2d(turtle_sorter *h, void *dev, const void *host, size_t bytes, hipStream_t st)
+{
+    int rc = ensure_staging(h);
+    if (rc) return rc;
+    int k = 0;
+    for (size_t off = 0; off < bytes; off += STAGE_BYTES, k ^= 1) {
+        const size_t len = bytes - off < STAGE_BYTES ? bytes - off : STAGE_BYTES;
+        if (h->stage_used[k]) HIPCHK(hipEventSynchronize(h->stage_ev[k]), "hipEventSynchronize");
+        memcpy(h->stage[k], (const char *)host + off, len);
+        HIPCHK(hipMemcpyAsync((char *)dev + off, h->stage[k], len, hipMemcpyHostToDevice, st), "H2D");
+        HIPCHK(hipEventRecord(h->stage_ev[k], st), "hipEventRecord");
+        h->stage_used[k] = true;
+    }
+    return TURTLE_OK;
+}
+
+// device -> host, pipelined the same way; returns when host holds the data
+int stage_d2h(turtle_sorter *h, void *host, const void *dev, size_t bytes, hipStream_t st)
+{
+    int rc = ensure_staging(h);
+    if (rc) return rc;
+    const size_t n = (bytes + STAGE_BYTES - 1) / STAGE_BYTES;
+    auto issue = [&](size_t i) -> int {
+        const int k = (int)(i & 1);
+        const size_t off = i * STAGE_BYTES;
+        const size_t len = bytes - off < STAGE_BYTES ? bytes - off : STAGE_BYTES;
+        if (h->stage_used[k]) HIPCHK(hipEventSynchronize(h->stage_ev[k]), "hipEventSynchronize");
+        HIPCHK(hipMemcpyAsync(h->stage[k], (const char *)dev + off, len, hipMemcpyDeviceToHost, st), "D2H");
+        HIPCHK(hipEventRecord(h->stage_ev[k], st), "hipEventRecord");
+        h->stage_used[k] = true;
+        return TURTLE_OK;
+    };
+    if (n > 0 && (rc = issue(0))) return rc;
+    for (size_t i = 0; i < n; i++) {
+        if (i + 1 < n && (rc = issue(i + 1))) return rc;
+        const int k = (int)(i & 1);
+        const size_t off = i * STAGE_BYTES;
+        const size_t len = bytes - off < STAGE_BYTES ? bytes - off : STAGE_BYTES;
+        HIPCHK(hipEventSynchronize(h->stage_ev[k]), "hipEventSynchronize");
+        memcpy((char *)host + off, h->stage[k], len);
+    }
+    return TURTLE_OK;
 }
 
 int alloc_buf(void **p, size_t bytes, const char *what)
@@ -392,6 +461,11 @@ int turtle_destroy(turtle_handle h)
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     free_tables(h);
+    if (h->ctrl_host) (void)hipHostFree(h->ctrl_host);
+    for (int k = 0; k < 2; k++) {
+        if (h->stage[k]) (void)hipHostFree(h->stage[k]);
+        if (h->stage_ev[k]) (void)hipEventDestroy(h->stage_ev[k]);
+    }
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -462,8 +536,8 @@ int turtle_segmented_sort_device(turtle_handle h, const void *keys_in, const int
     }
     h->host_off.resize(need);
     for (size_t s = 0; s < need; s++) h->host_off[s] = (uint32_t)offsets[s];
-    HIPCHK(hipMemcpyAsync(h->dev_off, h->host_off.data(), need * sizeof(uint32_t), hipMemcpyHostToDevice, st),
-           "hipMemcpyAsync(offsets)");
+    int src = stage_h2d(h, h->dev_off, h->host_off.data(), need * sizeof(uint32_t), st);
+    if (src) return src;
     return run_engine(h, keys_in, (const uint32_t *)idx_in, keys_out, (uint32_t *)idx_out, (uint64_t)total,
                       h->dev_off, (uint32_t)num_segments, st);
 }
@@ -525,11 +599,12 @@ int turtle_sort_start(turtle_handle h, void *keys, int32_t *idx, int64_t n)
     h->host_keys = keys;
     h->host_idx = idx;
     h->host_n = n;
+    int rc = TURTLE_OK;
     if (n > 0) {
-        HIPCHK(hipMemcpyAsync(h->kin, keys, (size_t)n * h->kb, hipMemcpyHostToDevice, h->stream), "H2D keys");
-        if (idx) HIPCHK(hipMemcpyAsync(h->iin, idx, (size_t)n * 4, hipMemcpyHostToDevice, h->stream), "H2D idx");
+        if ((rc = stage_h2d(h, h->kin, keys, (size_t)n * h->kb, h->stream))) return rc;
+        if (idx && (rc = stage_h2d(h, h->iin, idx, (size_t)n * 4, h->stream))) return rc;
     }
-    int rc = run_engine(h, h->kin, idx ? h->iin : nullptr, h->kout, idx ? h->iout : nullptr, (uint64_t)n, nullptr, 1,
+    rc = run_engine(h, h->kin, idx ? h->iin : nullptr, h->kout, idx ? h->iout : nullptr, (uint64_t)n, nullptr, 1,
                         h->stream);
     if (rc) return rc;
     h->started = true;
@@ -552,8 +627,8 @@ int turtle_sort_sync(turtle_handle h, double *seconds)
     }
     if (rc) return rc;
     if (h->host_n > 0) {
-        HIPCHK(hipMemcpy(h->host_keys, h->kout, (size_t)h->host_n * h->kb, hipMemcpyDeviceToHost), "D2H keys");
-        if (h->host_idx) HIPCHK(hipMemcpy(h->host_idx, h->iout, (size_t)h->host_n * 4, hipMemcpyDeviceToHost), "D2H idx");
+        if ((rc = stage_d2h(h, h->host_keys, h->kout, (size_t)h->host_n * h->kb, h->stream))) return rc;
+        if (h->host_idx && (rc = stage_d2h(h, h->host_idx, h->iout, (size_t)h->host_n * 4, h->stream))) return rc;
     }
     if (seconds) *seconds = now_s() - h->t_start;
     h->host_keys = nullptr;
@@ -575,14 +650,100 @@ int turtle_multisort(turtle_handle h, int dtype, int64_t arr_size, int block_siz
     const int64_t total = arr_size * num_arrays;
     if (total > 0) {
         if (!host_data) return fail(TURTLE_ERR_INVALID, "NULL host data");
-        HIPCHK(hipMemcpyAsync(h->kin, host_data, (size_t)total * h->kb, hipMemcpyHostToDevice, h->stream), "H2D");
-        int rc = turtle_multisort_device(h, dtype, arr_size, h->kin, h->kout, num_arrays, h->stream);
+        int rc = stage_h2d(h, h->kin, host_data, (size_t)total * h->kb, h->stream);
+        if (rc) return rc;
+        rc = turtle_multisort_device(h, dtype, arr_size, h->kin, h->kout, num_arrays, h->stream);
         if (rc) return rc;
         rc = turtle_synchronize(h, nullptr);
         if (rc) return rc;
-        HIPCHK(hipMemcpy(host_data, h->kout, (size_t)total * h->kb, hipMemcpyDeviceToHost), "D2H");
+        if ((rc = stage_d2h(h, host_data, h->kout, (size_t)total * h->kb, h->stream))) return rc;
     }
     if (seconds) *seconds = now_s() - t0;
+    return TURTLE_OK;
+}
+
+struct turtle_graph {
+    turtle_handle h = nullptr;
+    uint32_t *dev_off = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t total = 0;
+    uint32_t nseg = 0;
+};
+
+int turtle_graph_destroy(turtle_graph_handle g)
+{
+    if (!g) return TURTLE_OK;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->dev_off) (void)hipFree(g->dev_off);
+    delete g;
+    return TURTLE_OK;
+}
+
+int turtle_segmented_graph_create(turtle_handle h, const void *keys_in, const int32_t *idx_in, void *keys_out,
+                                  int32_t *idx_out, const int64_t *offsets, int64_t num_segments,
+                                  turtle_graph_handle *out)
+{
+    if (!out) return fail(TURTLE_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (!h) return fail(TURTLE_ERR_INVALID, "NULL handle");
+    if (num_segments <= 0 || !offsets) return fail(TURTLE_ERR_INVALID, "bad segment list");
+    if ((idx_in == nullptr) != (idx_out == nullptr)) return fail(TURTLE_ERR_INVALID, "idx_in and idx_out must both be set");
+    if (idx_in && !h->track) return fail(TURTLE_ERR_INVALID, "handle created without index tracking");
+    if (num_segments >= (int64_t)1 << 30) return fail(TURTLE_ERR_CAPACITY, "too many segments");
+    if (offsets[0] < 0) return fail(TURTLE_ERR_INVALID, "negative offset");
+    for (int64_t s = 0; s < num_segments; s++)
+        if (offsets[s + 1] < offsets[s]) return fail(TURTLE_ERR_INVALID, "offsets must be non-decreasing");
+    const int64_t total = offsets[num_segments];
+    if (total > h->max_n) return fail(TURTLE_ERR_CAPACITY, "segments exceed the handle's max_n");
+    if (total > 0 && (!keys_in || !keys_out)) return fail(TURTLE_ERR_INVALID, "NULL key buffer");
+    if (total > 0 && keys_in == keys_out) return fail(TURTLE_ERR_INVALID, "graph form needs keys_out != keys_in");
+    turtle_graph *g = new turtle_graph();
+    g->h = h;
+    g->total = (uint64_t)total;
+    g->nseg = (uint32_t)num_segments;
+    std::vector<uint32_t> off((size_t)num_segments + 1);
+    for (size_t s = 0; s < off.size(); s++) off[s] = (uint32_t)offsets[s];
+    int rc = ensure_tables(h, (uint64_t)total, (uint32_t)num_segments, 1);
+    hipError_t e = hipSuccess;
+    if (!rc) e = hipMalloc((void **)&g->dev_off, off.size() * sizeof(uint32_t));
+    if (!rc && e == hipSuccess) {
+        rc = stage_h2d(h, g->dev_off, off.data(), off.size() * sizeof(uint32_t), h->stream);
+        if (!rc) e = hipStreamSynchronize(h->stream);
+    }
+    hipStream_t cs = nullptr;
+    if (!rc && e == hipSuccess) e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+    if (!rc && e == hipSuccess) e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (!rc && e == hipSuccess) {
+        rc = run_engine(h, keys_in, (const uint32_t *)idx_in, keys_out, (uint32_t *)idx_out, (uint64_t)total,
+                        g->dev_off, (uint32_t)num_segments, cs, true);
+        hipGraph_t graph = nullptr;
+        hipError_t e2 = hipStreamEndCapture(cs, &graph);
+        g->graph = graph;
+        if (!rc && e2 != hipSuccess) e = e2;
+    }
+    if (!rc && e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (cs) (void)hipStreamDestroy(cs);
+    if (rc || e != hipSuccess) {
+        turtle_graph_destroy(g);
+        return rc ? rc : fail_hip(e, "graph capture");
+    }
+    *out = g;
+    return TURTLE_OK;
+}
+
+int turtle_graph_launch(turtle_graph_handle g, void *stream)
+{
+    if (!g) return fail(TURTLE_ERR_INVALID, "NULL graph");
+    hipStream_t st = stream_of(g->h, stream);
+    HIPCHK(hipGraphLaunch(g->exec, st), "hipGraphLaunch");
+    g->h->last_stream = st;
+    g->h->last_n = g->total;
+    g->h->last_engine = true;
+    g->h->last_total = g->total;
+    g->h->last_nroots = g->nseg;
+    g->h->timed = false;
     return TURTLE_OK;
 }
 

@@ -27,6 +27,10 @@ __device__ __forceinline__ uint32_t lane() { return __lane_id(); }
 
 __device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
+// orders LDS accesses of the lanes of one wave (read-all then leader-update
+// sequences on per-wave counters)
+__device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); }
+
 // number of set bits of m strictly below this lane
 __device__ __forceinline__ uint32_t mask_rank(uint64_t m)
 {

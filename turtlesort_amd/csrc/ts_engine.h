@@ -12,7 +12,8 @@ enum : uint32_t {
     ERR_SEGS = 2u,     // segment table full
     ERR_TILES = 4u,    // tile-count table full
     ERR_TIMEOUT = 8u,  // watchdog expired
-    ERR_BAD = 16u      // inconsistent task (should not happen)
+    ERR_BAD = 16u,     // inconsistent task (should not happen)
+    ERR_RANGE = 32u    // an index failed its range check; the access was skipped
 };
 
 // task kinds; a slot's type word also carries the source depth of LDS tasks
@@ -37,7 +38,8 @@ struct Ctrl {
     unsigned long long bytes_scatter;
     unsigned long long bytes_lds;
     unsigned long long tasks[5];   // indexed by task kind
-    uint32_t pad[4];
+    uint32_t bad_where;            // site code of the first failed range check
+    uint32_t pad[3];
 };
 
 // One queue entry.  gen is written last (release); a slot is valid for the
@@ -72,7 +74,7 @@ struct Params {
     uint32_t qcap, segcap, tilecap;
     uint32_t cutoff;          // LDS sort at or below this length
     uint32_t tiles_per_task;
-    uint32_t pad0;
+    uint32_t n_total;         // length of every key / index buffer
     uint64_t flip;            // sign-bit mask for signed keys, else 0
     uint64_t timeout_ticks;   // watchdog, in 100 MHz ticks
 };

@@ -153,6 +153,18 @@ __device__ __forceinline__ uint32_t classify(U x, const U (&spl)[S])
     return (le > lt) ? 2 * lt + 1 : 2 * lt;
 }
 
+// Records a failed range check (the caller skips the access); the run then
+// ends with ERR_RANGE instead of touching memory outside its buffers.
+__device__ __forceinline__ void range_fail(const Params &p, uint32_t where)
+{
+    if (atomic_or(&p.ctrl->error, ERR_RANGE) == 0) store_relaxed(&p.ctrl->bad_where, where);
+}
+
+__device__ __forceinline__ bool span_ok(const Params &p, uint32_t begin, uint32_t len)
+{
+    return len > 0 && begin <= p.n_total && len <= p.n_total - begin;
+}
+
 template <int KB, bool IDX, int S>
 struct Tasks {
     typedef Eng<KB, IDX, S> E;
@@ -171,6 +183,10 @@ struct Tasks {
         const U *src = src_keys<U>(p, depth);
         const uint32_t *isrc = src_idx(p, depth);
         U *dst = (U *)p.kout;
+        if (!span_ok(p, begin, len) || len > E::CAP) {
+            if (threadIdx.x == 0) range_fail(p, 1);
+            return;
+        }
         uint32_t n_eff = next_pow2_dev(len);
         if (n_eff < (uint32_t)SI) n_eff = SI;
         for (uint32_t i = threadIdx.x; i < n_eff; i += NT) {
@@ -191,8 +207,16 @@ struct Tasks {
     // ---------------------------------------------------------------- pivots
     static __device__ void pivot(const Params &p, Shared &sh, unsigned char *arena, uint32_t gen, uint32_t sid)
     {
+        if (sid >= p.segcap) {
+            if (threadIdx.x == 0) range_fail(p, 2);
+            return;
+        }
         Seg *sg = &p.segs[sid];
         const uint32_t begin = sg->begin, len = sg->len, depth = sg->depth;
+        if (!span_ok(p, begin, len)) {
+            if (threadIdx.x == 0) range_fail(p, 3);
+            return;
+        }
         const U flip = (U)p.flip;
         const U *src = src_keys<U>(p, depth);
         U *sk = (U *)arena;
@@ -244,9 +268,18 @@ struct Tasks {
     static __device__ void count(const Params &p, Shared &sh, unsigned char *arena, uint32_t gen, uint32_t sid,
                                  uint32_t task)
     {
+        if (sid >= p.segcap) {
+            if (threadIdx.x == 0) range_fail(p, 4);
+            return;
+        }
         Seg *sg = &p.segs[sid];
         const uint32_t begin = sg->begin, len = sg->len, depth = sg->depth;
         const uint32_t ntiles = sg->ntiles, tile_base = sg->tile_base;
+        if (!span_ok(p, begin, len) || ntiles != (len + TILE - 1) / TILE || tile_base > p.tilecap ||
+            ntiles > p.tilecap - tile_base || task * p.tiles_per_task >= ntiles) {
+            if (threadIdx.x == 0) range_fail(p, 5);
+            return;
+        }
         U spl[S];
         load_split(sg, spl);
         const U flip = (U)p.flip;
@@ -261,6 +294,7 @@ struct Tasks {
             const uint32_t tl = (len - ts) < TILE ? (len - ts) : TILE;
             nbytes += (uint64_t)tl * KB;
             if (lane() < (uint32_t)NB) sh.wcnt[w][lane()] = 0;
+            wave_sync();
             U x[TI];
 #pragma unroll
             for (int c = 0; c < TI; c++) {
@@ -351,9 +385,18 @@ struct Tasks {
     static __device__ void scatter(const Params &p, Shared &sh, unsigned char *arena, uint32_t gen, uint32_t sid,
                                    uint32_t task)
     {
+        if (sid >= p.segcap) {
+            if (threadIdx.x == 0) range_fail(p, 6);
+            return;
+        }
         Seg *sg = &p.segs[sid];
         const uint32_t begin = sg->begin, len = sg->len, depth = sg->depth;
         const uint32_t ntiles = sg->ntiles, tile_base = sg->tile_base;
+        if (!span_ok(p, begin, len) || ntiles != (len + TILE - 1) / TILE || tile_base > p.tilecap ||
+            ntiles > p.tilecap - tile_base || task * p.tiles_per_task >= ntiles) {
+            if (threadIdx.x == 0) range_fail(p, 7);
+            return;
+        }
         U spl[S];
         load_split(sg, spl);
         const U flip = (U)p.flip;
@@ -376,6 +419,7 @@ struct Tasks {
             const uint32_t tl = (len - ts) < TILE ? (len - ts) : TILE;
             nbytes += 2ull * tl * (KB + E::IB);
             if (lane() < (uint32_t)NB) sh.wcnt[w][lane()] = 0;
+            wave_sync();
             U x[TI];
             uint32_t ix[TI];
             uint32_t bpack[(TI + 3) / 4];
@@ -423,20 +467,30 @@ struct Tasks {
                 const bool valid = e < tl;
                 const uint32_t bin = (bpack[c / 4] >> (8 * (c % 4))) & 0xffu;
                 const uint64_t peers = match_peers<BITS>(bin, valid);
+                const uint32_t r = mask_rank(peers);
+                const uint32_t base = valid ? sh.woff[w][bin] : 0u;
+                wave_sync();                       // every lane has read before a leader updates
                 if (valid) {
-                    const uint32_t r = mask_rank(peers);
-                    const uint32_t base = sh.woff[w][bin];
                     const uint32_t pos = base + r;
-                    stk[pos] = x[c];
-                    if (IDX) sti[pos] = ix[c];
-                    stb[pos] = (unsigned char)bin;
+                    if (pos < tl) {
+                        stk[pos] = x[c];
+                        if (IDX) sti[pos] = ix[c];
+                        stb[pos] = (unsigned char)bin;
+                    } else {
+                        range_fail(p, 8);
+                    }
                     if (r == 0) sh.woff[w][bin] = base + (uint32_t)__popcll(peers);
                 }
+                wave_sync();
             }
             __syncthreads();
             for (uint32_t i = threadIdx.x; i < tl; i += NT) {
                 const uint32_t b = stb[i];
-                const uint32_t d = sh.dbase[b] + i;
+                const uint32_t d = b < (uint32_t)NB ? sh.dbase[b] + i : 0xffffffffu;
+                if (d - begin >= len) {          // outside the segment (also catches b >= NB)
+                    range_fail(p, 9);
+                    continue;
+                }
                 const U kv = stk[i];
                 if (b & 1u) {
                     kfin[d] = kv;
@@ -568,6 +622,7 @@ __global__ void ts_reset(Ctrl *c)
         c->tile_alloc = 0;
         c->error = 0;
         c->max_depth = 0;
+        c->bad_where = 0;
         c->bytes_count = 0;
         c->bytes_scatter = 0;
         c->bytes_lds = 0;
@@ -587,6 +642,10 @@ __global__ void ts_roots(Params p, const uint32_t *off, uint32_t nseg, uint32_t 
     if (s < nseg) {
         beg = off ? off[s] : 0u;
         len = off ? off[s + 1] - beg : single_len;
+    }
+    if (len > 0 && !span_ok(p, beg, len)) {
+        range_fail(p, 10);
+        len = 0;
     }
     const bool has = len > 0;
     const bool big = len > p.cutoff;
