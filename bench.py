@@ -6,12 +6,20 @@ A step is one sort of 67,108,864 int32 keys + int32 index (uniform over
 HBM before the timed region and are not modified by the sort
 (turtle_sort_device, keys_in -> keys_out).  Prints one JSON line.
 
-value      = keys sorted per second over the K timed steps (Gkeys/s), wall clock
-             between two torch.cuda.synchronize() calls
-roofline   = algorithmic bytes of the passes the engine executed (device
-             counters: count passes K B/key, partition scatters and LDS sorts
-             2(K+I) B/key) / average duration of one sort (HIP events on the
-             stream the kernels run on), against the 8 TB/s HBM3E peak
+value      = keys sorted per second over the K timed steps (Gkeys/s), wall
+             clock between two torch.cuda.synchronize() calls.  Launch timing
+             is on during those steps (an event pair per kernel launch);
+             "value_untimed_launches" repeats the steps without it.
+roofline   = the dominant kernel (largest summed device time over the timed
+             steps, from the per-launch HIP events recorded on the stream the
+             kernels run on): its algorithmic bytes (device counters: count
+             pass K B/key, partition scatter and LDS sort 2(K+I) B/key) over
+             its summed duration, against the 8 TB/s HBM3E peak.  "traffic"
+             is the HBM bytes per launch measured with rocprofv3 counters,
+             read from the committed profile summary when present.
+end_to_end = the same algorithmic bytes of all passes over the whole sort's
+             device time (the "fraction of HBM roofline across the actual
+             number of passes")
 cpu_baseline = the oracle's restatement of the reference algorithm (one
              core) on a bounded sample of the same distribution
 """
@@ -38,13 +46,26 @@ def parse():
     return ap.parse_args()
 
 
-def time_sorts(torch, sorter, kin, kout, iin, iout, steps, warmup):
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_scatter_64M_int32_idx_traffic.md")
+# bytes of the algorithmic model per kernel kind (device counters)
+ALG_BYTES_OF = {"count": "bytes_count", "scatter": "bytes_scatter", "lds": "bytes_lds"}
+KERNEL_NAMES = {"count": "k_count<4,true,3>", "scatter": "k_scatter<4,true,3>", "lds": "k_lds<4,true,3>",
+                "pivot": "k_pivot<4,true,3>", "scan": "k_scan<4,true,3>", "children": "k_children<4,true,3>",
+                "level": "k_level", "reset": "k_reset", "roots": "k_roots"}
+
+
+def time_sorts(torch, sorter, kin, kout, iin, iout, steps, warmup, profile=True):
+    """K timed sorts on torch's current stream (the stream the library runs
+    on); returns wall seconds, per-step device ms, the last step's counters
+    and, with profile, the per-kernel sums over exactly the timed steps."""
     for _ in range(warmup):
         sorter.sort_device(kin, kout, iin, iout)
     sorter.synchronize()
     torch.cuda.synchronize()
+    sorter.set_profiling(profile)
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
     ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(steps):
         starts[i].record()
@@ -52,9 +73,40 @@ def time_sorts(torch, sorter, kin, kout, iin, iout, steps, warmup):
         ends[i].record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    prof = sorter.get_profile() if profile else None
+    sorter.set_profiling(False)
     st = sorter.synchronize()   # counters of the last step (every step sorts the same input)
     dev_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
-    return wall, dev_ms, st
+    return wall, dev_ms, st, prof
+
+
+def read_traffic():
+    """HBM bytes per launch of the scatter kernel from the committed
+    rocprofv3 counter summary (line 'traffic_bytes_per_launch: N')"""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.strip().startswith("traffic_bytes_per_launch:"):
+                    return int(float(line.split(":", 1)[1].strip()))
+    except OSError:
+        pass
+    return None
+
+
+def kernel_table(prof, st, steps):
+    """per kernel kind: launches, ms per step, GB/s of its algorithmic bytes"""
+    out = {}
+    for kind, rec in prof.items():
+        if rec["launches"] == 0:
+            continue
+        row = {"launches_per_step": rec["launches"] / steps, "ms_per_step": round(rec["ms"] / steps, 4),
+               "avg_launch_ms": round(rec["ms"] / rec["launches"], 5)}
+        if kind in ALG_BYTES_OF:
+            b = st[ALG_BYTES_OF[kind]]
+            row["alg_bytes_per_step"] = int(b)
+            row["GB_s"] = round(b * steps / (rec["ms"] * 1e-3) / 1e9, 1) if rec["ms"] > 0 else None
+        out[kind] = row
+    return out
 
 
 def check_sorted(torch, kin, kout, iin, iout):
@@ -90,14 +142,23 @@ def main():
     fill_iota(iin)
     torch.cuda.synchronize()
 
-    wall, dev_ms, st = time_sorts(torch, sorter, kin, kout, iin, iout, args.steps, args.warmup)
+    wall, dev_ms, st, prof = time_sorts(torch, sorter, kin, kout, iin, iout, args.steps, args.warmup)
     valid = check_sorted(torch, kin, kout, iin, iout)
     ms_step = wall * 1e3 / args.steps
     value = n * args.steps / wall / 1e9
+    wall_u, _, _, _ = time_sorts(torch, sorter, kin, kout, iin, iout, args.steps, 0, profile=False)
     avg_dev_ms = sum(dev_ms) / len(dev_ms)
     alg_bytes = st["bytes_count"] + st["bytes_scatter"] + st["bytes_lds"]
-    achieved = alg_bytes / (avg_dev_ms * 1e-3) / 1e9
     floor_bytes = 2 * n * (kb + ib)
+    kernels = kernel_table(prof, st, args.steps)
+    # dominant kernel: largest summed device time among the kinds that move
+    # the keys (the bookkeeping kernels have no byte model)
+    dom = max((k for k in kernels if k in ALG_BYTES_OF), key=lambda k: kernels[k]["ms_per_step"])
+    drow = kernels[dom]
+    d_bytes_launch = drow["alg_bytes_per_step"] / drow["launches_per_step"]
+    d_achieved = d_bytes_launch / (drow["avg_launch_ms"] * 1e-3) / 1e9
+    traffic = read_traffic() if dom == "scatter" else None
+    e2e = alg_bytes / (avg_dev_ms * 1e-3) / 1e9
 
     out = {
         "metric": "Gkeys/s + %HBM peak @64M int32+idx; batched vs plain keys/s; p99 ms",
@@ -116,19 +177,27 @@ def main():
                                "uniform [0,2^31)", "n": n, "key_bytes": kb, "index_bytes": ib,
                    "tuning": sorter.get_tuning()},
         "valid": valid,
+        "value_untimed_launches": round(n * args.steps / wall_u / 1e9, 4),
         "roofline": {
             "bound": "hbm",
-            "achieved": round(achieved, 1),
+            "achieved": round(d_achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "kernel": "ts_drain<4,true,3> (persistent sort engine; events cover reset+roots+drain)",
-            "alg_bytes_per_launch": int(alg_bytes),
-            "avg_launch_ms": round(avg_dev_ms, 4),
+            "frac": round(d_achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "kernel": KERNEL_NAMES[dom],
+            "alg_bytes_per_launch": int(d_bytes_launch),
+            "avg_launch_ms": drow["avg_launch_ms"],
+        },
+        "end_to_end": {
+            "alg_bytes_per_sort": int(alg_bytes),
+            "avg_sort_ms": round(avg_dev_ms, 4),
+            "achieved_GB_s": round(e2e, 1),
+            "frac": round(e2e / HBM_PEAK_GBS, 4),
             "passes_equiv": round(alg_bytes / floor_bytes, 3),
             "floor_frac": round(floor_bytes / (avg_dev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         },
+        "kernels": kernels,
         "engine": {k: st[k] for k in ("bytes_count", "bytes_scatter", "bytes_lds", "tasks_pivot", "tasks_count",
                                       "tasks_scatter", "tasks_lds", "segments", "max_depth")},
     }
@@ -144,7 +213,7 @@ def main():
             else:
                 fill_keys(kin, wl.SEED_C2_FEW, fill)
             torch.cuda.synchronize()
-            w2, d2, s2 = time_sorts(torch, sorter, kin, kout, iin, iout, args.steps, 1)
+            w2, d2, s2, _ = time_sorts(torch, sorter, kin, kout, iin, iout, args.steps, 1, profile=False)
             ok2 = check_sorted(torch, kin, kout, iin, iout)
             b2 = s2["bytes_count"] + s2["bytes_scatter"] + s2["bytes_lds"]
             a2 = sum(d2) / len(d2)

@@ -30,7 +30,7 @@ typedef enum turtle_status {
     TURTLE_ERR_UNSUPPORTED = 5,    /* feature not available on this device */
     TURTLE_ERR_DEVICE = 6,         /* HIP runtime error */
     TURTLE_ERR_QUEUE_OVERFLOW = 7, /* device work queue / segment table full */
-    TURTLE_ERR_TIMEOUT = 8,        /* persistent kernel watchdog expired */
+    TURTLE_ERR_TIMEOUT = 8,        /* reserved: no kernel of this library waits on another */
     TURTLE_ERR_STATE = 9,          /* call out of order (Sync without StartSorting...) */
     TURTLE_ERR_BUDGET = 10         /* request larger than the serving memory pool */
 } turtle_status;
@@ -63,10 +63,12 @@ typedef struct turtle_stats {
 /* Tunables of the sort engine (chosen by the autotuner, or set by hand). */
 typedef struct turtle_tuning {
     int32_t splitters;         /* pivots per partition: 3 (the reference's 3-pivot
-                                  partition) or 15 (two 3-pivot levels per pass) */
+                                  partition; the only value accepted) */
     int32_t lds_cutoff;        /* segments at or below this length are sorted in LDS */
-    int32_t tiles_per_task;    /* queue granularity: tiles claimed per queue slot */
-    int32_t workgroups_per_cu; /* persistent grid = CUs x this (0 = occupancy limit) */
+    int32_t tiles_per_task;    /* tiles a workgroup takes per step of the count and
+                                  scatter loops */
+    int32_t workgroups_per_cu; /* grid of the tile kernels = CUs x this (0 = occupancy
+                                  limit) */
 } turtle_tuning;
 
 /* ---- sorter object: Turtle::TurtleSort<Type,TrackIndex> ------------------ */
@@ -112,18 +114,22 @@ int turtle_multisort(turtle_handle h, int dtype, int64_t arr_size, int block_siz
 /* Device half of StartSorting+Sync: replaces the wave chain
  * copyTasksBack -> resetNumTasks -> quickSortWithoutStreamCompaction/bruteSort
  * (turtle-sort.cu:35-188, 573-934) and the chunk merge of Sync
- * (turtle-sort.cu:190-300, turtle-sort.cuh:192-243) with one persistent
- * kernel draining a device work queue.  Sorts keys_in[0..n) into keys_out
- * (and carries idx_in into idx_out when both are non-NULL; the handle must
- * track indices).  keys_in is not modified; keys_out may equal keys_in.
- * stream is a hipStream_t (NULL = the handle's stream).  Asynchronous. */
+ * (turtle-sort.cu:190-300, turtle-sort.cuh:192-243) with level-synchronous
+ * partition passes over a device-resident segment list followed by one LDS
+ * sort launch.  Sorts keys_in[0..n) into keys_out (and carries idx_in into
+ * idx_out when both are non-NULL; the handle must track indices).  keys_in
+ * is not modified; keys_out may equal keys_in.  stream is a hipStream_t;
+ * NULL is the null (default) stream, so the work is ordered after whatever
+ * produced the inputs there.  Returns once the work is queued; the host
+ * waits for a small read-back between batches of levels. */
 int turtle_sort_device(turtle_handle h, const void *keys_in, const int32_t *idx_in,
                        void *keys_out, int32_t *idx_out, int64_t n, void *stream);
 
-/* Batched uneven segments in one persistent launch (no reference
+/* Batched uneven segments in one engine run (no reference
  * counterpart; the reference's only route is a loop of StartSorting+Sync).
  * Segment s is [offsets[s], offsets[s+1]) of the concatenated arrays;
- * offsets is a HOST array of num_segments+1 entries.  Asynchronous. */
+ * offsets is a HOST array of num_segments+1 entries.  Stream rules as
+ * turtle_sort_device. */
 int turtle_segmented_sort_device(turtle_handle h, const void *keys_in, const int32_t *idx_in,
                                  void *keys_out, int32_t *idx_out,
                                  const int64_t *offsets, int64_t num_segments, void *stream);
@@ -134,10 +140,10 @@ int turtle_segmented_sort_device(turtle_handle h, const void *keys_in, const int
 int turtle_multisort_device(turtle_handle h, int dtype, int64_t arr_size,
                             const void *keys_in, void *keys_out, int64_t num_arrays, void *stream);
 
-/* hipGraph form of turtle_segmented_sort_device: the reset, root and
- * persistent-drain launches are captured once (offsets copied to a device
- * buffer owned by the graph; no memset nodes) and replayed by
- * turtle_graph_launch on any stream.  The graph uses the handle's engine
+/* hipGraph form of turtle_segmented_sort_device: the reset, root, level
+ * and LDS-sort launches are captured once for a fixed number of levels
+ * (offsets copied to a device buffer owned by the graph; no memset nodes)
+ * and replayed by turtle_graph_launch on any stream (NULL = null stream).  The graph uses the handle's engine
  * tables, so at most one run per handle may be in flight.  keys_out must
  * differ from keys_in.  No reference counterpart. */
 typedef struct turtle_graph *turtle_graph_handle;
@@ -149,6 +155,27 @@ int turtle_graph_destroy(turtle_graph_handle g);
 
 /* Waits for the handle's last device work and fills *st (may be NULL). */
 int turtle_synchronize(turtle_handle h, turtle_stats *st);
+
+/* Kernels of one engine run (index of turtle_profile's arrays). */
+typedef enum turtle_kernel_kind {
+    TURTLE_K_RESET = 0, TURTLE_K_ROOTS = 1, TURTLE_K_LEVEL = 2, TURTLE_K_PIVOT = 3,
+    TURTLE_K_COUNT = 4, TURTLE_K_SCAN = 5, TURTLE_K_SCATTER = 6, TURTLE_K_CHILDREN = 7,
+    TURTLE_K_LDS = 8, TURTLE_K_KINDS = 9
+} turtle_kernel_kind;
+
+typedef struct turtle_profile {
+    double ms[TURTLE_K_KINDS];          /* summed device time per kernel kind */
+    uint64_t launches[TURTLE_K_KINDS];  /* launches per kernel kind */
+} turtle_profile;
+
+/* Per-kernel launch timing (no reference counterpart: the reference times
+ * only the whole sort, in Sync).  While profiling is on, every engine run
+ * records a HIP event pair around each kernel launch, on the stream the
+ * kernel runs on (graph captures record none).  turtle_get_profile waits for
+ * those events, returns the sums since the previous call (or since profiling
+ * was switched on) and starts new sums. */
+int turtle_set_profiling(turtle_handle h, int on);
+int turtle_get_profile(turtle_handle h, turtle_profile *out);
 
 /* Tuning of a handle (defaults are the autotuned table, or built-ins). */
 int turtle_get_tuning(turtle_handle h, turtle_tuning *t);

@@ -1,4 +1,4 @@
-"""Batched uneven segments in one persistent launch: each segment equal to
+"""Batched uneven segments in one engine run: each segment equal to
 the oracle's sort of it (keys bit-exact, stable index)."""
 import numpy as np
 import pytest

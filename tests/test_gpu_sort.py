@@ -136,3 +136,32 @@ def test_device_form_repeat_and_inplace(torch_dev):
     sorter.synchronize()
     assert torch.equal(k2, outs[0][0]) and torch.equal(i2, outs[0][1])
     sorter.close()
+
+
+@pytest.mark.parametrize("n", [3_000_000, 8_000_000])
+def test_device_form_repeats_exact(torch_dev, n):
+    """Regression test for two fixed races.  (1) The outputs are prefilled on
+    the null stream right before each sort: the sort must run after that fill
+    (NULL stream = the null stream), so no position may keep the fill value.
+    (2) Every run must give torch's stable order exactly: a block barrier
+    that did not wait for LDS loads let the merge's index gather read values
+    of the next merge round, which duplicated a few indices per run."""
+    torch = torch_dev
+    from turtlesort_amd import fill_keys, fill_iota
+    sorter = TurtleSort(np.int32, n)
+    kin = torch.empty(n, dtype=torch.int32, device="cuda")
+    iin = torch.empty(n, dtype=torch.int32, device="cuda")
+    fill_keys(kin, 5 + n, 33)
+    fill_iota(iin)
+    ref_k, ref_i = torch.sort(kin.to(torch.int64), stable=True)
+    for rep in range(4):
+        print(f"case n={n} rep={rep}", flush=True)
+        ko = torch.full_like(kin, -7)
+        io = torch.full_like(iin, -7)
+        sorter.sort_device(kin, ko, iin, io)
+        st = sorter.synchronize()
+        assert st["error"] == 0
+        assert torch.equal(ko.to(torch.int64), ref_k), f"keys differ on run {rep}"
+        bad = int((io.to(torch.int64) != ref_i).sum())
+        assert bad == 0, f"{bad} indices differ from the stable order on run {rep}"
+    sorter.close()

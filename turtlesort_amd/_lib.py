@@ -65,6 +65,20 @@ class Tuning(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+# kernel kinds (turtle_kernel_kind)
+KERNEL_KINDS = ("reset", "roots", "level", "pivot", "count", "scan", "scatter", "children", "lds")
+
+
+class Profile(ctypes.Structure):
+    _fields_ = [
+        ("ms", ctypes.c_double * len(KERNEL_KINDS)),
+        ("launches", ctypes.c_uint64 * len(KERNEL_KINDS)),
+    ]
+
+    def as_dict(self):
+        return {k: {"ms": self.ms[i], "launches": self.launches[i]} for i, k in enumerate(KERNEL_KINDS)}
+
+
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int
 _i64 = ctypes.c_int64
@@ -85,6 +99,8 @@ SIGNATURES = {
     "turtle_segmented_graph_create": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, ctypes.POINTER(_vp)]),
     "turtle_graph_launch": (_i32, [_vp, _vp]),
     "turtle_graph_destroy": (_i32, [_vp]),
+    "turtle_set_profiling": (_i32, [_vp, _i32]),
+    "turtle_get_profile": (_i32, [_vp, ctypes.POINTER(Profile)]),
     "turtle_get_tuning": (_i32, [_vp, ctypes.POINTER(Tuning)]),
     "turtle_set_tuning": (_i32, [_vp, ctypes.POINTER(Tuning)]),
     "turtle_fill_keys": (_i32, [_i32, _vp, _i64, _u64, _i32, _vp]),

@@ -152,6 +152,16 @@ class TurtleSort:
         _lib.check(self._lib.turtle_synchronize(self._h, ctypes.byref(st)))
         return st.as_dict()
 
+    # ---- per-kernel launch timing ---------------------------------------------------
+    def set_profiling(self, on=True):
+        _lib.check(self._lib.turtle_set_profiling(self._h, int(bool(on))))
+
+    def get_profile(self):
+        """{kernel kind: {"ms": summed device ms, "launches": n}} since the last call"""
+        pr = _lib.Profile()
+        _lib.check(self._lib.turtle_get_profile(self._h, ctypes.byref(pr)))
+        return pr.as_dict()
+
     # ---- tuning -------------------------------------------------------------------
     def get_tuning(self):
         t = _lib.Tuning()

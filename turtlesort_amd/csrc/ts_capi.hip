@@ -18,6 +18,10 @@
 #include "ts_engine.h"
 #include "ts_internal.h"
 
+static_assert((int)ts::K_KINDS == (int)TURTLE_K_KINDS && (int)ts::K_SCATTER == (int)TURTLE_K_SCATTER &&
+                  (int)ts::K_LDS == (int)TURTLE_K_LDS,
+              "kernel kinds of the engine and the C ABI differ");
+
 namespace {
 
 thread_local std::string g_last_error;
@@ -73,6 +77,61 @@ uint64_t env_u64(const char *name, uint64_t dflt)
     return strtoull(v, nullptr, 10);
 }
 
+// Launch timing: an event pair per kernel launch, taken from a pool that
+// grows on demand and is reused after every turtle_get_profile.
+struct EventTimer : ts::LaunchTimer {
+    struct Rec {
+        int kind;
+        hipEvent_t a, b;
+    };
+    std::vector<hipEvent_t> pool;
+    std::vector<Rec> recs;
+    size_t used = 0;
+    hipError_t err = hipSuccess;
+
+    hipEvent_t take()
+    {
+        if (used == pool.size()) {
+            hipEvent_t e = nullptr;
+            const hipError_t r = hipEventCreate(&e);
+            if (r != hipSuccess) {
+                err = r;
+                return nullptr;
+            }
+            pool.push_back(e);
+        }
+        return pool[used++];
+    }
+    void begin(int kind, hipStream_t st) override
+    {
+        if (err != hipSuccess) return;
+        hipEvent_t a = take(), b = take();
+        if (!a || !b) return;
+        const hipError_t r = hipEventRecord(a, st);
+        if (r != hipSuccess) {
+            err = r;
+            return;
+        }
+        recs.push_back({kind, a, b});
+    }
+    void end(int kind, hipStream_t st) override
+    {
+        if (err != hipSuccess || recs.empty() || recs.back().kind != kind) return;
+        const hipError_t r = hipEventRecord(recs.back().b, st);
+        if (r != hipSuccess) err = r;
+    }
+    void reset()
+    {
+        recs.clear();
+        used = 0;
+        err = hipSuccess;
+    }
+    ~EventTimer() override
+    {
+        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    }
+};
+
 }  // namespace
 
 struct turtle_sorter {
@@ -90,10 +149,13 @@ struct turtle_sorter {
     // engine tables
     ts::Ctrl *ctrl = nullptr;
     ts::Ctrl *ctrl_host = nullptr;   // pinned readback copy
-    ts::Slot *q = nullptr;
     ts::Seg *segs = nullptr;
     uint32_t *tiles = nullptr;
-    uint32_t qcap = 0, segcap = 0, tilecap = 0;
+    uint32_t *tile_seg = nullptr;
+    ts::LdsTask *ldst = nullptr;
+    ts::Level *levels = nullptr;
+    ts::Level *levels_host = nullptr;  // pinned readback (a Level followed by a Ctrl)
+    uint32_t segcap = 0, tilecap = 0, ldscap = 0;
     uint32_t *dev_off = nullptr;
     size_t dev_off_cap = 0;
     std::vector<uint32_t> host_off;
@@ -116,62 +178,62 @@ struct turtle_sorter {
     bool last_engine = false;
     uint64_t last_total = 0;
     uint32_t last_nroots = 0;
+    uint32_t levels_run = 0;
+    // per-kernel launch timing (turtle_set_profiling)
+    EventTimer *timer = nullptr;
 };
 
 namespace {
 
 void free_tables(turtle_sorter *h)
 {
-    if (h->q) (void)hipFree(h->q);
-    if (h->segs) (void)hipFree(h->segs);
-    if (h->tiles) (void)hipFree(h->tiles);
-    h->q = nullptr;
+    void *bufs[] = {h->segs, h->tiles, h->tile_seg, h->ldst, h->levels};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
     h->segs = nullptr;
     h->tiles = nullptr;
-    h->qcap = h->segcap = h->tilecap = 0;
+    h->tile_seg = nullptr;
+    h->ldst = nullptr;
+    h->levels = nullptr;
+    h->segcap = h->tilecap = h->ldscap = 0;
 }
 
-// Sizes the queue, segment and tile tables for a run over `total` keys in
-// `nroots` root segments.  Bounds: segments longer than the cutoff are
-// disjoint at each depth, so about (total/cutoff) per depth; tiles are
-// ceil(len/TILE) per segment; tasks are one pivot, count and scatter tasks,
-// and at most S+1 children per segment.  `scale` grows them after an
-// overflow.  Table memory is reused while large enough.
+// Sizes the segment, tile and LDS-task tables for a run over `total` keys
+// in `nroots` root segments.  Segments longer than the cutoff are disjoint
+// within a level, so a level holds fewer than total/cutoff of them; with
+// sample-quantile pivots each level shrinks them about S+1 times, so all
+// levels together stay near 1.4 total/cutoff (8x that is allocated).  Tiles
+// are per level: at most total/TILE + (segments of the level).  Every
+// segment yields at most S+1 LDS tasks.  `scale` grows the tables after an
+// overflow.  TURTLE_SEGCAP overrides the segment capacity (tests force the
+// overflow path with it).
 int ensure_tables(turtle_sorter *h, uint64_t total, uint32_t nroots, uint32_t scale)
 {
     const uint64_t cutoff = (uint64_t)h->tune.lds_cutoff;
     const uint64_t tile = ts::engine_tile(h->kb);
     const uint64_t S = (uint64_t)h->tune.splitters;
-    uint64_t segcap = (4 * (total / cutoff + 1) + 1024) * scale + nroots;
-    uint64_t tilecap = (8 * (total / tile + 1)) * scale + segcap;
-    uint64_t qcap = segcap * (S + 2) + 2 * tilecap + nroots + 1024;
-    const uint64_t cap_q = env_u64("TURTLE_QUEUE_CAP", 0);
-    if (cap_q) qcap = cap_q;
-    if (qcap > 0xffffff00ull || segcap > 0xffffff00ull || tilecap > 0xffffff00ull)
+    uint64_t segcap = (8 * (total / cutoff + 1) + 1024) * scale + nroots;
+    const uint64_t force = env_u64("TURTLE_SEGCAP", 0);
+    if (force) segcap = force;
+    const uint64_t tilecap = (total / tile + 1) + segcap;
+    const uint64_t ldscap = segcap * (S + 1) + nroots + 1024;
+    if (segcap > 0xffffff00ull || tilecap > 0xffffff00ull || ldscap > 0xffffff00ull)
         return fail(TURTLE_ERR_CAPACITY, "work tables would exceed 2^32 entries");
-    if (h->q && h->qcap >= qcap && h->segcap >= segcap && h->tilecap >= tilecap && !cap_q) return TURTLE_OK;
-    if (h->q && cap_q && h->qcap == qcap && h->segcap >= segcap && h->tilecap >= tilecap) return TURTLE_OK;
+    if (h->segs && h->segcap >= segcap && h->tilecap >= tilecap && h->ldscap >= ldscap &&
+        (!force || h->segcap == segcap))
+        return TURTLE_OK;
     // the old tables may still be in use by a run in flight
-    if (h->q) HIPCHK(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    if (h->segs) HIPCHK(hipDeviceSynchronize(), "hipDeviceSynchronize");
     free_tables(h);
-    HIPCHK(hipMalloc((void **)&h->q, qcap * sizeof(ts::Slot)), "hipMalloc(queue)");
-    HIPCHK(hipMemset(h->q, 0, qcap * sizeof(ts::Slot)), "hipMemset(queue)");
     HIPCHK(hipMalloc((void **)&h->segs, segcap * sizeof(ts::Seg)), "hipMalloc(segments)");
     HIPCHK(hipMalloc((void **)&h->tiles, tilecap * ts::MAX_BINS * sizeof(uint32_t)), "hipMalloc(tiles)");
-    h->qcap = (uint32_t)qcap;
+    HIPCHK(hipMalloc((void **)&h->tile_seg, tilecap * sizeof(uint32_t)), "hipMalloc(tile map)");
+    HIPCHK(hipMalloc((void **)&h->ldst, ldscap * sizeof(ts::LdsTask)), "hipMalloc(lds tasks)");
+    HIPCHK(hipMalloc((void **)&h->levels, ts::MAX_LEVELS * sizeof(ts::Level)), "hipMalloc(levels)");
     h->segcap = (uint32_t)segcap;
     h->tilecap = (uint32_t)tilecap;
-    // the queue was re-created with gen 0 everywhere; the control block keeps
-    // counting generations, which start above 0 after the first reset
+    h->ldscap = (uint32_t)ldscap;
     return TURTLE_OK;
-}
-
-int grid_of(turtle_sorter *h, bool idx)
-{
-    int occ = ts::engine_occupancy(h->kb, idx, h->tune.splitters);
-    int per_cu = h->tune.workgroups_per_cu > 0 && h->tune.workgroups_per_cu < occ ? h->tune.workgroups_per_cu : occ;
-    int g = h->cus * per_cu;
-    return g > 0 ? g : 1;
 }
 
 ts::Params make_params(turtle_sorter *h, const void *kin, const uint32_t *iin, void *kout, uint32_t *iout)
@@ -186,34 +248,65 @@ ts::Params make_params(turtle_sorter *h, const void *kin, const uint32_t *iin, v
     p.it0 = h->it0;
     p.it1 = h->it1;
     p.ctrl = h->ctrl;
-    p.q = h->q;
     p.segs = h->segs;
     p.tiles = h->tiles;
-    p.qcap = h->qcap;
+    p.tile_seg = h->tile_seg;
+    p.ldst = h->ldst;
+    p.levels = h->levels;
     p.segcap = h->segcap;
     p.tilecap = h->tilecap;
+    p.ldscap = h->ldscap;
     p.cutoff = (uint32_t)h->tune.lds_cutoff;
-    p.tiles_per_task = (uint32_t)(h->tune.tiles_per_task > 0 ? h->tune.tiles_per_task : 1);
+    p.tiles_per_wg = (uint32_t)(h->tune.tiles_per_task > 0 ? h->tune.tiles_per_task : 1);
     p.n_total = 0;   // set by run_engine
     p.flip = h->flip;
-    const uint64_t ms = env_u64("TURTLE_WATCHDOG_MS", 20000);
-    p.timeout_ticks = ms * 100000ull;   // wall_clock64 runs at 100 MHz
     return p;
 }
 
+// partition levels expected for a segment of n keys: the longest segment
+// shrinks about (S+1) times per level until it fits the cutoff; two spare
+// levels cover uneven splits
+uint32_t levels_estimate(uint64_t n, uint64_t cutoff, uint64_t S)
+{
+    uint32_t L = 0;
+    while (n > cutoff && L < 64) {
+        n = (n + S) / (S + 1);
+        L++;
+    }
+    return L + 2;
+}
+
+int ensure_readback(turtle_sorter *h)
+{
+    if (!h->levels_host)
+        HIPCHK(hipHostMalloc((void **)&h->levels_host, sizeof(ts::Level) + sizeof(ts::Ctrl), hipHostMallocDefault),
+               "hipHostMalloc");
+    return TURTLE_OK;
+}
+
+// Device forms run on the caller's stream; NULL is the null stream (as in
+// every HIP call), never the handle's private non-blocking stream, which is
+// not ordered after the null-stream work that produced the inputs.
 hipStream_t stream_of(turtle_sorter *h, void *stream)
 {
-    return stream ? (hipStream_t)stream : h->stream;
+    (void)h;
+    return (hipStream_t)stream;
 }
 
 // One engine run over device buffers.  In-place requests (kin == kout) first
 // copy the input into the second ping-pong buffer, which depth 0 only reads.
+// The levels are enqueued in batches; after a batch one small pinned
+// readback tells whether segments are left (none for the usual inputs after
+// the first batch).  A graph capture cannot read back, so it captures a
+// generous fixed number of levels and turtle_synchronize reports ERR_DEPTH
+// if that was not enough.
 int run_engine(turtle_sorter *h, const void *kin, const uint32_t *iin, void *kout, uint32_t *iout, uint64_t total,
                const uint32_t *dev_off, uint32_t nseg, hipStream_t st, bool capturing = false)
 {
     const bool idx = iin != nullptr && iout != nullptr;
     int rc = ensure_tables(h, total, nseg, 1);
     if (rc) return rc;
+    if ((rc = ensure_readback(h))) return rc;
     if (total > 0 && kin == kout) {
         HIPCHK(hipMemcpyAsync(h->kt1, kin, total * h->kb, hipMemcpyDeviceToDevice, st), "hipMemcpyAsync(in-place keys)");
         kin = h->kt1;
@@ -224,9 +317,49 @@ int run_engine(turtle_sorter *h, const void *kin, const uint32_t *iin, void *kou
     }
     ts::Params p = make_params(h, kin, idx ? iin : nullptr, kout, idx ? iout : nullptr);
     p.n_total = (uint32_t)total;
+    const int S = h->tune.splitters, wpc = h->tune.workgroups_per_cu;
+    // the longest root decides the level count
+    uint64_t longest = total;
+    if (dev_off && h->host_off.size() > 1) {
+        longest = 0;
+        for (size_t s = 0; s + 1 < h->host_off.size(); s++) {
+            const uint64_t l = (uint64_t)h->host_off[s + 1] - h->host_off[s];
+            if (l > longest) longest = l;
+        }
+    }
+    uint32_t nlev = levels_estimate(longest, (uint64_t)h->tune.lds_cutoff, (uint64_t)S);
     if (!capturing) HIPCHK(hipEventRecord(h->ev0, st), "hipEventRecord");
-    HIPCHK(ts::engine_run(p, h->kb, idx, h->tune.splitters, grid_of(h, idx), dev_off, nseg, (uint32_t)total, st),
-           "engine launch");
+    if (capturing) {
+        nlev += 6;
+        if (nlev > (uint32_t)ts::MAX_LEVELS) nlev = ts::MAX_LEVELS;
+        HIPCHK(ts::engine_levels(p, h->kb, idx, S, h->cus, wpc, dev_off, nseg, (uint32_t)total, 0, nlev, true, true, st),
+               "engine launch");
+        h->levels_run = nlev;
+    } else {
+        uint32_t L0 = 0;
+        bool first = true;
+        for (;;) {
+            if (L0 + nlev > (uint32_t)ts::MAX_LEVELS) nlev = ts::MAX_LEVELS - L0;
+            HIPCHK(ts::engine_levels(p, h->kb, idx, S, h->cus, wpc, dev_off, nseg, (uint32_t)total, L0, nlev, first,
+                                     false, st, h->timer),
+                   "engine launch");
+            first = false;
+            L0 += nlev;
+            // segments created by the last level are the ones still to do
+            ts::Ctrl *ch = (ts::Ctrl *)(h->levels_host + 1);
+            HIPCHK(hipMemcpyAsync(h->levels_host, h->levels + (L0 - 1), sizeof(ts::Level), hipMemcpyDeviceToHost, st),
+                   "D2H(level)");
+            HIPCHK(hipMemcpyAsync(ch, h->ctrl, sizeof(ts::Ctrl), hipMemcpyDeviceToHost, st), "D2H(ctrl)");
+            HIPCHK(hipStreamSynchronize(st), "hipStreamSynchronize");
+            const uint32_t left = ch->seg_alloc > h->levels_host->se ? ch->seg_alloc - h->levels_host->se : 0u;
+            if (left == 0 || ch->error != 0 || L0 >= (uint32_t)ts::MAX_LEVELS) break;
+            nlev = 4;
+        }
+        h->levels_run = L0;
+        HIPCHK(ts::engine_levels(p, h->kb, idx, S, h->cus, wpc, dev_off, nseg, (uint32_t)total, L0, 0, false, true, st,
+                                 h->timer),
+               "engine launch");
+    }
     if (!capturing) HIPCHK(hipEventRecord(h->ev1, st), "hipEventRecord");
     h->timed = !capturing;
     h->last_stream = st;
@@ -240,9 +373,17 @@ int run_engine(turtle_sorter *h, const void *kin, const uint32_t *iin, void *kou
 int read_stats(turtle_sorter *h, turtle_stats *st, uint32_t *err_out)
 {
     if (!h->ctrl_host) HIPCHK(hipHostMalloc((void **)&h->ctrl_host, sizeof(ts::Ctrl), hipHostMallocDefault), "hipHostMalloc");
+    int rc = ensure_readback(h);
+    if (rc) return rc;
     HIPCHK(hipMemcpyAsync(h->ctrl_host, h->ctrl, sizeof(ts::Ctrl), hipMemcpyDeviceToHost, h->last_stream), "D2H(ctrl)");
+    if (h->last_engine && h->levels_run > 0)
+        HIPCHK(hipMemcpyAsync(h->levels_host, h->levels + (h->levels_run - 1), sizeof(ts::Level), hipMemcpyDeviceToHost,
+                              h->last_stream), "D2H(level)");
     HIPCHK(hipStreamSynchronize(h->last_stream), "hipStreamSynchronize");
     const ts::Ctrl c = *h->ctrl_host;
+    uint32_t err = h->last_engine ? c.error : 0u;
+    // segments created by the last level were never partitioned
+    if (h->last_engine && h->levels_run > 0 && c.seg_alloc > h->levels_host->se) err |= ts::ERR_DEPTH;
     float ms = 0.f;
     if (h->timed) HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1), "hipEventElapsedTime");
     if (st) {
@@ -259,22 +400,22 @@ int read_stats(turtle_sorter *h, turtle_stats *st, uint32_t *err_out)
             st->tasks_lds = c.tasks[ts::T_LDS];
             st->segments = c.seg_alloc;
             st->max_depth = c.max_depth;
-            st->error = c.error;
         }
+        st->error = err;
     }
-    *err_out = h->last_engine ? c.error : 0u;
+    *err_out = err;
     return TURTLE_OK;
 }
 
 int error_status(uint32_t err)
 {
     if (err == 0) return TURTLE_OK;
-    if (err & ts::ERR_TIMEOUT) return fail(TURTLE_ERR_TIMEOUT, "persistent sort kernel watchdog expired");
-    if (err & ts::ERR_RANGE)
-        return fail(TURTLE_ERR_DEVICE, "device range check failed (access skipped)");
-    if (err & (ts::ERR_QUEUE | ts::ERR_SEGS | ts::ERR_TILES))
-        return fail(TURTLE_ERR_QUEUE_OVERFLOW, "device work queue or segment/tile table overflow");
-    return fail(TURTLE_ERR_DEVICE, "inconsistent partition detected on the device");
+    if (err & ts::ERR_RANGE) return fail(TURTLE_ERR_DEVICE, "device range check failed (access skipped)");
+    if (err & ts::ERR_BAD) return fail(TURTLE_ERR_DEVICE, "inconsistent partition detected on the device");
+    if (err & (ts::ERR_SEGS | ts::ERR_TILES | ts::ERR_LDSQ))
+        return fail(TURTLE_ERR_QUEUE_OVERFLOW, "device work list (segments/tiles/LDS tasks) overflow");
+    if (err & ts::ERR_DEPTH) return fail(TURTLE_ERR_QUEUE_OVERFLOW, "segments left after the last partition level");
+    return fail(TURTLE_ERR_DEVICE, "device error");
 }
 
 bool valid_tuning(const turtle_tuning &t, int kb)
@@ -456,12 +597,15 @@ int turtle_create(int dtype, int64_t max_n, int track_index, int compress, turtl
 int turtle_destroy(turtle_handle h)
 {
     if (!h) return TURTLE_OK;
+    if (h->last_stream != h->stream) (void)hipStreamSynchronize(h->last_stream);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    delete h->timer;
     void *bufs[] = {h->kin, h->kout, h->kt0, h->kt1, h->iin, h->iout, h->it0, h->it1, h->ctrl, h->dev_off};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     free_tables(h);
     if (h->ctrl_host) (void)hipHostFree(h->ctrl_host);
+    if (h->levels_host) (void)hipHostFree(h->levels_host);
     for (int k = 0; k < 2; k++) {
         if (h->stage[k]) (void)hipHostFree(h->stage[k]);
         if (h->stage_ev[k]) (void)hipEventDestroy(h->stage_ev[k]);
@@ -477,6 +621,42 @@ int turtle_compression_enabled(turtle_handle h)
 {
     (void)h;
     return 0;
+}
+
+int turtle_set_profiling(turtle_handle h, int on)
+{
+    if (!h) return fail(TURTLE_ERR_INVALID, "NULL handle");
+    if (on && !h->timer) h->timer = new EventTimer();
+    if (!on && h->timer) {
+        HIPCHK(hipStreamSynchronize(h->last_stream), "hipStreamSynchronize");
+        delete h->timer;
+        h->timer = nullptr;
+    }
+    return TURTLE_OK;
+}
+
+int turtle_get_profile(turtle_handle h, turtle_profile *out)
+{
+    if (!h || !out) return fail(TURTLE_ERR_INVALID, "NULL argument");
+    memset(out, 0, sizeof(*out));
+    if (!h->timer) return fail(TURTLE_ERR_STATE, "profiling is off");
+    EventTimer *t = h->timer;
+    if (t->err != hipSuccess) {
+        const hipError_t e = t->err;
+        t->reset();
+        return fail_hip(e, "launch timing");
+    }
+    if (!t->recs.empty()) HIPCHK(hipEventSynchronize(t->recs.back().b), "hipEventSynchronize");
+    for (const EventTimer::Rec &r : t->recs) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, r.a, r.b), "hipEventElapsedTime");
+        if (r.kind >= 0 && r.kind < TURTLE_K_KINDS) {
+            out->ms[r.kind] += ms;
+            out->launches[r.kind] += 1;
+        }
+    }
+    t->reset();
+    return TURTLE_OK;
 }
 
 int turtle_get_tuning(turtle_handle h, turtle_tuning *t)
@@ -564,7 +744,7 @@ int turtle_multisort_device(turtle_handle h, int dtype, int64_t arr_size, const 
         h->last_engine = false;
         return TURTLE_OK;
     }
-    // longer arrays: equal segments through the persistent engine
+    // longer arrays: equal segments through the sort engine
     const int64_t total = arr_size * num_arrays;
     if (total > h->max_n) return fail(TURTLE_ERR_CAPACITY, "arrays exceed the handle's max_n");
     std::vector<int64_t> off((size_t)num_arrays + 1);
@@ -705,6 +885,7 @@ int turtle_segmented_graph_create(turtle_handle h, const void *keys_in, const in
     g->nseg = (uint32_t)num_segments;
     std::vector<uint32_t> off((size_t)num_segments + 1);
     for (size_t s = 0; s < off.size(); s++) off[s] = (uint32_t)offsets[s];
+    h->host_off = off;   // the level estimate reads the longest segment from here
     int rc = ensure_tables(h, (uint64_t)total, (uint32_t)num_segments, 1);
     hipError_t e = hipSuccess;
     if (!rc) e = hipMalloc((void **)&g->dev_off, off.size() * sizeof(uint32_t));

@@ -31,6 +31,18 @@ __device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_bal
 // sequences on per-wave counters)
 __device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); }
 
+// Block barrier that first waits for this wave's LDS loads to return
+// (s_waitcnt lgkmcnt(0); the vmcnt and expcnt fields keep their maximum, so
+// global loads are not waited for).  The compiler emits s_barrier without
+// that wait when a load's result is only used after the barrier, and on
+// gfx950 such a load in flight has returned a value that another wave
+// stored after the barrier (a write-after-read race on the LDS buffer).
+__device__ __forceinline__ void block_sync()
+{
+    __builtin_amdgcn_s_waitcnt(0xc07f);
+    __syncthreads();
+}
+
 // number of set bits of m strictly below this lane
 __device__ __forceinline__ uint32_t mask_rank(uint64_t m)
 {
@@ -119,7 +131,7 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *red, u
     const int w = threadIdx.x / WAVE;
     uint32_t inc = wave_incl_scan(v);
     if (lane() == WAVE - 1) red[w] = inc;
-    __syncthreads();
+    block_sync();
     uint32_t wpre = 0, tot = 0;
 #pragma unroll
     for (int i = 0; i < NW; i++) {
@@ -127,7 +139,7 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *red, u
         wpre += (i < w) ? s : 0u;
         tot += s;
     }
-    __syncthreads();
+    block_sync();
     *total = tot;
     return wpre + inc - v;
 }
@@ -182,7 +194,7 @@ __device__ void block_sort(U *sk, uint32_t *si, uint32_t n_eff)
         }
     }
     for (uint32_t run = ITEMS; run < n_eff; run *= 2) {
-        __syncthreads();
+        block_sync();
         if (active) {
 #pragma unroll
             for (int i = 0; i < ITEMS; i++) {
@@ -190,7 +202,7 @@ __device__ void block_sort(U *sk, uint32_t *si, uint32_t n_eff)
                 if (IDX) si[pidx<ITEMS>(t * ITEMS + i)] = v[i];
             }
         }
-        __syncthreads();
+        block_sync();
         if (active) {
             const uint32_t pos = t * ITEMS;
             const uint32_t base = pos & ~(2 * run - 1);
@@ -231,7 +243,7 @@ __device__ void block_sort(U *sk, uint32_t *si, uint32_t n_eff)
             }
         }
     }
-    __syncthreads();
+    block_sync();
     if (active) {
 #pragma unroll
         for (int i = 0; i < ITEMS; i++) {
@@ -239,7 +251,7 @@ __device__ void block_sort(U *sk, uint32_t *si, uint32_t n_eff)
             if (IDX) si[pidx<ITEMS>(t * ITEMS + i)] = v[i];
         }
     }
-    __syncthreads();
+    block_sync();
 }
 
 __host__ __device__ constexpr uint32_t next_pow2(uint32_t x)

@@ -1,6 +1,6 @@
-// ts_engine.h -- host-visible definitions of the persistent sort engine:
-// the device control block, queue slots, segment records and the launch
-// parameters shared by ts_engine.hip and the C ABI.
+// ts_engine.h -- host-visible definitions of the sort engine: the device
+// control block, per-level work lists, segment records and the launch
+// parameters shared by ts_levels.hip and the C ABI.
 #pragma once
 #include <stdint.h>
 
@@ -8,53 +8,55 @@ namespace ts {
 
 // error flags raised on the device (Ctrl::error)
 enum : uint32_t {
-    ERR_QUEUE = 1u,    // work queue full
     ERR_SEGS = 2u,     // segment table full
-    ERR_TILES = 4u,    // tile-count table full
-    ERR_TIMEOUT = 8u,  // watchdog expired
-    ERR_BAD = 16u,     // inconsistent task (should not happen)
-    ERR_RANGE = 32u    // an index failed its range check; the access was skipped
+    ERR_TILES = 4u,    // tile table full
+    ERR_LDSQ = 8u,     // LDS-sort task list full
+    ERR_BAD = 16u,     // bin totals do not add up to the segment length
+    ERR_RANGE = 32u,   // an index failed its range check; the access was skipped
+    ERR_DEPTH = 64u    // segments were still left after the last level
 };
 
-// task kinds; a slot's type word also carries the source depth of LDS tasks
+// task kinds (statistics index)
 enum : uint32_t { T_PIVOT = 1, T_COUNT = 2, T_SCATTER = 3, T_LDS = 4 };
+
+// kernel kinds of one engine run (launch timing index; the C ABI's
+// turtle_kernel_kind has the same values)
+enum : int { K_RESET = 0, K_ROOTS, K_LEVEL, K_PIVOT, K_COUNT, K_SCAN, K_SCATTER, K_CHILDREN, K_LDS, K_KINDS };
 
 constexpr int MAX_SPLIT = 15;
 constexpr int MAX_BINS = 2 * MAX_SPLIT + 1;
+constexpr int MAX_LEVELS = 64;
 
-// Device control block (one per handle).  Reset before every run by the
-// reset kernel; gen increases by one per run so stale queue slots of earlier
-// runs are never taken for new work.
+// Device control block (one per handle), cleared by the reset kernel before
+// every run.
 struct Ctrl {
-    uint32_t gen;
-    uint32_t head;          // next queue slot to claim
-    uint32_t tail;          // next queue slot to fill
-    int32_t outstanding;    // tasks pushed and not yet finished
-    uint32_t seg_alloc;
-    uint32_t tile_alloc;
+    uint32_t seg_alloc;     // segments created so far (all levels)
+    uint32_t tile_alloc;    // tiles of the current level
+    uint32_t lds_alloc;     // LDS-sort tasks created so far
     uint32_t error;
     uint32_t max_depth;
+    uint32_t bad_where;     // site code of the first failed range check
+    uint32_t pad0, pad1;
     unsigned long long bytes_count;
     unsigned long long bytes_scatter;
     unsigned long long bytes_lds;
     unsigned long long tasks[5];   // indexed by task kind
-    uint32_t bad_where;            // site code of the first failed range check
-    uint32_t pad[3];
 };
 
-// One queue entry.  gen is written last (release); a slot is valid for the
-// current run when gen matches Ctrl::gen.
-struct Slot {
-    uint32_t gen;
-    uint32_t type;
-    uint32_t a;
-    uint32_t b;
+// Segments of level L are the ids [sb, se) of the segment table.
+struct Level {
+    uint32_t sb, se, tiles, pad;
+};
+
+// A segment at or below the LDS cutoff, sorted in one workgroup.
+struct LdsTask {
+    uint32_t begin, len, depth, pad;
 };
 
 // A segment being partitioned (length above the LDS cutoff).
 struct Seg {
     uint32_t begin, len, depth, ntiles;
-    uint32_t tile_base, ntasks, count_pending, scatter_pending;
+    uint32_t tile_base, pad0, pad1, pad2;
     uint64_t split[MAX_SPLIT];     // splitters in order representation, ascending
     uint32_t base[MAX_BINS + 1];   // first position of each bin inside the segment
     uint32_t total[MAX_BINS + 1];  // elements per bin
@@ -68,15 +70,17 @@ struct Params {
     void *kt0, *kt1;          // ping-pong keys
     uint32_t *it0, *it1;      // ping-pong indices
     Ctrl *ctrl;
-    Slot *q;
     Seg *segs;
     uint32_t *tiles;          // per-tile bin counts, then per-tile bin prefixes
-    uint32_t qcap, segcap, tilecap;
+    uint32_t *tile_seg;       // segment id of every tile of the current level
+    LdsTask *ldst;
+    Level *levels;
+    uint32_t segcap, tilecap, ldscap, pad0;
     uint32_t cutoff;          // LDS sort at or below this length
-    uint32_t tiles_per_task;
+    uint32_t tiles_per_wg;    // tiles a workgroup takes per step of the tile loops
     uint32_t n_total;         // length of every key / index buffer
+    uint32_t pad1;
     uint64_t flip;            // sign-bit mask for signed keys, else 0
-    uint64_t timeout_ticks;   // watchdog, in 100 MHz ticks
 };
 
 }  // namespace ts

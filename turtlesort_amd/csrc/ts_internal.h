@@ -6,16 +6,25 @@
 
 namespace ts {
 
-// persistent engine (ts_engine.hip)
-int engine_occupancy(int kb, bool idx, int splitters);
+// Optional launch timing: begin/end are called on the host right before and
+// right after every kernel launch of an engine run (kind = K_*), so that an
+// implementation can record HIP events on the same stream.
+struct LaunchTimer {
+    virtual void begin(int kind, hipStream_t st) = 0;
+    virtual void end(int kind, hipStream_t st) = 0;
+    virtual ~LaunchTimer() {}
+};
+
+// level-synchronous engine (ts_levels.hip)
 bool engine_supports(int splitters);   // splitter counts compiled in
-uint32_t engine_tile(int kb);      // elements per partition tile
-uint32_t engine_cap(int kb);       // largest segment the LDS sort takes
-// Launches reset + roots + drain on st.  dev_off: nseg+1 segment offsets
-// (uint32) in device memory, or nullptr for the single segment
-// [0, single_len) (nseg must then be 1).
-hipError_t engine_run(const Params &p, int kb, bool idx, int splitters, int grid,
-                      const uint32_t *dev_off, uint32_t nseg, uint32_t single_len, hipStream_t st);
+uint32_t engine_tile(int kb);          // elements per partition tile
+uint32_t engine_cap(int kb);           // largest segment the LDS sort takes
+// Enqueues on st: (first) reset + roots, then levels [L0, L0+nlev), then
+// (last) the LDS sorts.  dev_off: nseg+1 segment offsets (uint32) in device
+// memory, or nullptr for the single segment [0, single_len).
+hipError_t engine_levels(const Params &p, int kb, bool idx, int splitters, int cus, int wg_per_cu,
+                         const uint32_t *dev_off, uint32_t nseg, uint32_t single_len, uint32_t L0, uint32_t nlev,
+                         bool first, bool last, hipStream_t st, LaunchTimer *timer = nullptr);
 
 // MultiSort kernels (ts_multisort.hip): arrays of arr_size keys, in -> out.
 // Returns hipErrorInvalidValue when arr_size exceeds multisort_max_size(kb).
