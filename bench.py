@@ -43,15 +43,25 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-variants", action="store_true", help="skip the ascending / 16-distinct runs")
     ap.add_argument("--cpu-sample", type=int, default=8 * 1024 * 1024)
+    ap.add_argument("--splitters", type=int, default=0, help="override the engine's splitter count")
+    ap.add_argument("--cutoff", type=int, default=0, help="override the LDS-sort cutoff")
+    ap.add_argument("--tiles", type=int, default=0, help="override the tiles per workgroup step")
     return ap.parse_args()
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_scatter_64M_int32_idx_traffic.md")
 # bytes of the algorithmic model per kernel kind (device counters)
 ALG_BYTES_OF = {"count": "bytes_count", "scatter": "bytes_scatter", "lds": "bytes_lds"}
-KERNEL_NAMES = {"count": "k_count<4,true,3>", "scatter": "k_scatter<4,true,3>", "lds": "k_lds<4,true,3>",
-                "pivot": "k_pivot<4,true,3>", "scan": "k_scan<4,true,3>", "children": "k_children<4,true,3>",
-                "level": "k_level", "reset": "k_reset", "roots": "k_roots"}
+
+
+def kernel_name(kind, S):
+    """device kernel behind a timing kind, for 4-byte keys with an index"""
+    t = f"<4,true,{S}>"
+    names = {"count": "k_count", "scatter": "k_scatter", "lds": "k_lds", "pivot": "k_pivot",
+             "scan": "k_scan1+k_scan2", "children": "k_children", "ldsprep": "k_ldscount+k_ldsplan+k_ldsbucket"}
+    if kind in names:
+        return "+".join(f"ts::{k}{t}" if k != "k_ldsplan" else f"ts::{k}" for k in names[kind].split("+"))
+    return f"ts::k_{kind}"
 
 
 def time_sorts(torch, sorter, kin, kout, iin, iout, steps, warmup, profile=True):
@@ -134,6 +144,10 @@ def main():
     n = wl.C2_N
     kb, ib = 4, 4
     sorter = TurtleSort(np.int32, n)
+    over = {k: v for k, v in (("splitters", args.splitters), ("lds_cutoff", args.cutoff),
+                              ("tiles_per_task", args.tiles)) if v}
+    if over:
+        sorter.set_tuning(**over)
     kin = torch.empty(n, dtype=torch.int32, device="cuda")
     iin = torch.empty(n, dtype=torch.int32, device="cuda")
     kout = torch.empty_like(kin)
@@ -185,7 +199,7 @@ def main():
             "unit": "GB/s",
             "frac": round(d_achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "kernel": KERNEL_NAMES[dom],
+            "kernel": kernel_name(dom, sorter.get_tuning()["splitters"]),
             "alg_bytes_per_launch": int(d_bytes_launch),
             "avg_launch_ms": drow["avg_launch_ms"],
         },

@@ -62,8 +62,8 @@ typedef struct turtle_stats {
 
 /* Tunables of the sort engine (chosen by the autotuner, or set by hand). */
 typedef struct turtle_tuning {
-    int32_t splitters;         /* pivots per partition: 3 (the reference's 3-pivot
-                                  partition; the only value accepted) */
+    int32_t splitters;         /* pivots per partition pass: 3 (the reference's
+                                  3-pivot partition), 7, 15 or 31 */
     int32_t lds_cutoff;        /* segments at or below this length are sorted in LDS */
     int32_t tiles_per_task;    /* tiles a workgroup takes per step of the count and
                                   scatter loops */
@@ -160,7 +160,7 @@ int turtle_synchronize(turtle_handle h, turtle_stats *st);
 typedef enum turtle_kernel_kind {
     TURTLE_K_RESET = 0, TURTLE_K_ROOTS = 1, TURTLE_K_LEVEL = 2, TURTLE_K_PIVOT = 3,
     TURTLE_K_COUNT = 4, TURTLE_K_SCAN = 5, TURTLE_K_SCATTER = 6, TURTLE_K_CHILDREN = 7,
-    TURTLE_K_LDS = 8, TURTLE_K_KINDS = 9
+    TURTLE_K_LDS = 8, TURTLE_K_LDSPREP = 9, TURTLE_K_KINDS = 10
 } turtle_kernel_kind;
 
 typedef struct turtle_profile {

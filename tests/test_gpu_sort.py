@@ -165,3 +165,22 @@ def test_device_form_repeats_exact(torch_dev, n):
         bad = int((io.to(torch.int64) != ref_i).sum())
         assert bad == 0, f"{bad} indices differ from the stable order on run {rep}"
     sorter.close()
+
+
+@pytest.mark.parametrize("splitters", [3, 7, 15, 31])
+def test_tuning_variants(orc, torch_dev, splitters):
+    """Every splitter count and a few LDS cutoffs / tile granularities give
+    the oracle's keys and the stable index order."""
+    sorter = TurtleSort(np.int32, 600_000)
+    for cutoff, tiles in [(8192, 1), (1000, 2), (16, 4)]:
+        sorter.set_tuning(splitters=splitters, lds_cutoff=cutoff, tiles_per_task=tiles)
+        for dist in ["full", "dup15", "asc", "few16"]:
+            n = 600_000 if cutoff > 16 else 70_000
+            print(f"case S={splitters} cutoff={cutoff} tiles={tiles} dist={dist} n={n}", flush=True)
+            keys = make_keys(np.int32, n, dist, 11 + cutoff)
+            k = keys.copy()
+            idx = np.arange(n, dtype=np.int32)
+            sorter.start_sorting(k, idx)
+            sorter.sync()
+            check_against_oracle(orc, keys, k, idx)
+    sorter.close()

@@ -66,7 +66,7 @@ class Tuning(ctypes.Structure):
 
 
 # kernel kinds (turtle_kernel_kind)
-KERNEL_KINDS = ("reset", "roots", "level", "pivot", "count", "scan", "scatter", "children", "lds")
+KERNEL_KINDS = ("reset", "roots", "level", "pivot", "count", "scan", "scatter", "children", "lds", "ldsprep")
 
 
 class Profile(ctypes.Structure):

@@ -150,9 +150,10 @@ struct turtle_sorter {
     ts::Ctrl *ctrl = nullptr;
     ts::Ctrl *ctrl_host = nullptr;   // pinned readback copy
     ts::Seg *segs = nullptr;
-    uint32_t *tiles = nullptr;
+    uint32_t *tcnt = nullptr, *tpre = nullptr, *spart = nullptr;
     uint32_t *tile_seg = nullptr;
     ts::LdsTask *ldst = nullptr;
+    uint32_t *ldsq = nullptr;
     ts::Level *levels = nullptr;
     ts::Level *levels_host = nullptr;  // pinned readback (a Level followed by a Ctrl)
     uint32_t segcap = 0, tilecap = 0, ldscap = 0;
@@ -187,13 +188,14 @@ namespace {
 
 void free_tables(turtle_sorter *h)
 {
-    void *bufs[] = {h->segs, h->tiles, h->tile_seg, h->ldst, h->levels};
+    void *bufs[] = {h->segs, h->tcnt, h->tpre, h->spart, h->tile_seg, h->ldst, h->ldsq, h->levels};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     h->segs = nullptr;
-    h->tiles = nullptr;
+    h->tcnt = h->tpre = h->spart = nullptr;
     h->tile_seg = nullptr;
     h->ldst = nullptr;
+    h->ldsq = nullptr;
     h->levels = nullptr;
     h->segcap = h->tilecap = h->ldscap = 0;
 }
@@ -226,9 +228,13 @@ int ensure_tables(turtle_sorter *h, uint64_t total, uint32_t nroots, uint32_t sc
     if (h->segs) HIPCHK(hipDeviceSynchronize(), "hipDeviceSynchronize");
     free_tables(h);
     HIPCHK(hipMalloc((void **)&h->segs, segcap * sizeof(ts::Seg)), "hipMalloc(segments)");
-    HIPCHK(hipMalloc((void **)&h->tiles, tilecap * ts::MAX_BINS * sizeof(uint32_t)), "hipMalloc(tiles)");
+    const uint64_t entries = tilecap * ts::MAX_BINS;
+    HIPCHK(hipMalloc((void **)&h->tcnt, entries * sizeof(uint32_t)), "hipMalloc(tile counts)");
+    HIPCHK(hipMalloc((void **)&h->tpre, entries * sizeof(uint32_t)), "hipMalloc(tile prefixes)");
+    HIPCHK(hipMalloc((void **)&h->spart, (entries / ts::SCAN_CHUNK + 2) * sizeof(uint32_t)), "hipMalloc(scan sums)");
     HIPCHK(hipMalloc((void **)&h->tile_seg, tilecap * sizeof(uint32_t)), "hipMalloc(tile map)");
     HIPCHK(hipMalloc((void **)&h->ldst, ldscap * sizeof(ts::LdsTask)), "hipMalloc(lds tasks)");
+    HIPCHK(hipMalloc((void **)&h->ldsq, ldscap * sizeof(uint32_t)), "hipMalloc(lds order)");
     HIPCHK(hipMalloc((void **)&h->levels, ts::MAX_LEVELS * sizeof(ts::Level)), "hipMalloc(levels)");
     h->segcap = (uint32_t)segcap;
     h->tilecap = (uint32_t)tilecap;
@@ -249,9 +255,12 @@ ts::Params make_params(turtle_sorter *h, const void *kin, const uint32_t *iin, v
     p.it1 = h->it1;
     p.ctrl = h->ctrl;
     p.segs = h->segs;
-    p.tiles = h->tiles;
+    p.tcnt = h->tcnt;
+    p.tpre = h->tpre;
+    p.spart = h->spart;
     p.tile_seg = h->tile_seg;
     p.ldst = h->ldst;
+    p.ldsq = h->ldsq;
     p.levels = h->levels;
     p.segcap = h->segcap;
     p.tilecap = h->tilecap;
@@ -264,16 +273,17 @@ ts::Params make_params(turtle_sorter *h, const void *kin, const uint32_t *iin, v
 }
 
 // partition levels expected for a segment of n keys: the longest segment
-// shrinks about (S+1) times per level until it fits the cutoff; two spare
-// levels cover uneven splits
+// shrinks about (S+1) times per level (a quarter more is allowed for uneven
+// splits) until it fits the cutoff, plus one spare level; the host reads
+// back after these and adds levels only if segments are left
 uint32_t levels_estimate(uint64_t n, uint64_t cutoff, uint64_t S)
 {
     uint32_t L = 0;
     while (n > cutoff && L < 64) {
-        n = (n + S) / (S + 1);
+        n = (n * 5 / 4 + S) / (S + 1);
         L++;
     }
-    return L + 2;
+    return L + 1;
 }
 
 int ensure_readback(turtle_sorter *h)
@@ -430,7 +440,7 @@ bool valid_tuning(const turtle_tuning &t, int kb)
 turtle_tuning default_tuning(int kb)
 {
     turtle_tuning t;
-    t.splitters = 3;
+    t.splitters = 31;
     t.lds_cutoff = (int32_t)ts::engine_cap(kb);
     t.tiles_per_task = 1;
     t.workgroups_per_cu = 0;

@@ -156,16 +156,19 @@ __host__ __device__ constexpr uint32_t padded_len(uint32_t n) { return n + n / I
 // ---- stable LDS block sort ---------------------------------------------------------
 // Sorts (order-value, payload) pairs held in LDS: sk[] keys in order
 // representation, si[] 32-bit payloads (ignored when !IDX), both in the pidx
-// layout.  n_eff is a power of two, ITEMS <= n_eff <= NT*ITEMS, and elements
-// [n, n_eff) have been filled with key_max (they stay behind every real
-// element because every step is stable).  Threads >= n_eff/ITEMS only take
-// part in the barriers.
+// layout.  The n_eff elements are n_eff / run_limit independent arrays of
+// run_limit elements each (run_limit == n_eff: one array).  n_eff and
+// run_limit are powers of two, ITEMS <= run_limit <= n_eff <= NT*ITEMS, and
+// the unused tail of every array has been filled with key_max (it stays
+// behind every real element because every step is stable).  Threads >=
+// n_eff/ITEMS only take part in the barriers.
 // Step 1: each thread sorts its ITEMS consecutive elements in registers with
 //         odd-even transposition (stable: swaps only strictly greater pairs);
-// Step 2: log2(n_eff/ITEMS) rounds of stable merge-path merges.
+// Step 2: log2(run_limit/ITEMS) rounds of stable merge-path merges; a merge
+//         never crosses a multiple of run_limit.
 // The result is left in LDS in the same layout.
 template <typename U, bool IDX, int NT, int ITEMS>
-__device__ void block_sort(U *sk, uint32_t *si, uint32_t n_eff)
+__device__ __forceinline__ void block_sort(U *sk, uint32_t *si, uint32_t n_eff, uint32_t run_limit)
 {
     const uint32_t t = threadIdx.x;
     const bool active = t * ITEMS < n_eff;
@@ -193,7 +196,7 @@ __device__ void block_sort(U *sk, uint32_t *si, uint32_t n_eff)
             }
         }
     }
-    for (uint32_t run = ITEMS; run < n_eff; run *= 2) {
+    for (uint32_t run = ITEMS; run < run_limit; run *= 2) {
         block_sync();
         if (active) {
 #pragma unroll
@@ -204,38 +207,57 @@ __device__ void block_sort(U *sk, uint32_t *si, uint32_t n_eff)
         }
         block_sync();
         if (active) {
-            const uint32_t pos = t * ITEMS;
-            const uint32_t base = pos & ~(2 * run - 1);
-            const uint32_t diag = pos - base;
-            const uint32_t A = base, B = base + run;
-            // merge path: number of A elements among the first diag outputs
-            uint32_t lo = diag > run ? diag - run : 0;
-            uint32_t hi = diag < run ? diag : run;
-            while (lo < hi) {
-                uint32_t mid = (lo + hi) >> 1;
-                U bk = sk[pidx<ITEMS>(B + diag - 1 - mid)];
-                U ak = sk[pidx<ITEMS>(A + mid)];
-                if (!(bk < ak)) lo = mid + 1; else hi = mid;
+            // The thread's ITEMS outputs are CH independent merge-path
+            // chains of ITEMS/CH outputs each: their binary searches and
+            // serial merges have no dependency on each other, so their LDS
+            // latencies overlap.
+            constexpr int CH = ITEMS >= 8 ? 2 : 1;
+            constexpr int PER = ITEMS / CH;
+            uint32_t ai[CH], bi[CH], aend[CH], bend[CH];
+            U ka[CH], kb[CH];
+#pragma unroll
+            for (int h = 0; h < CH; h++) {
+                const uint32_t pos = t * ITEMS + h * PER;
+                const uint32_t base = pos & ~(2 * run - 1);
+                const uint32_t diag = pos - base;
+                const uint32_t A = base, B = base + run;
+                // merge path: number of A elements among the first diag outputs
+                uint32_t lo = diag > run ? diag - run : 0;
+                uint32_t hi = diag < run ? diag : run;
+                while (lo < hi) {
+                    uint32_t mid = (lo + hi) >> 1;
+                    U bk = sk[pidx<ITEMS>(B + diag - 1 - mid)];
+                    U ak = sk[pidx<ITEMS>(A + mid)];
+                    if (!(bk < ak)) lo = mid + 1; else hi = mid;
+                }
+                ai[h] = A + lo;
+                bi[h] = B + (diag - lo);
+                aend[h] = B;
+                bend[h] = B + run;
             }
-            // serial stable merge of ITEMS outputs; ties take A.  One LDS
-            // load per step (the successor of the element taken); source
-            // positions are kept so the payloads are gathered afterwards.
-            // Positions A+run and B+run stay inside the padded array.
-            uint32_t ai = A + lo, bi = B + (diag - lo);
-            const uint32_t aend = B, bend = B + run;
-            U ka = sk[pidx<ITEMS>(ai)];
-            U kb = sk[pidx<ITEMS>(bi)];
+            // serial stable merges; ties take A.  One LDS load per step (the
+            // successor of the element taken); source positions are kept so
+            // the payloads are gathered afterwards.  Positions A+run and
+            // B+run stay inside the padded array.
+#pragma unroll
+            for (int h = 0; h < CH; h++) {
+                ka[h] = sk[pidx<ITEMS>(ai[h])];
+                kb[h] = sk[pidx<ITEMS>(bi[h])];
+            }
             uint32_t src[ITEMS];
 #pragma unroll
-            for (int i = 0; i < ITEMS; i++) {
-                const bool takeA = (bi >= bend) || (ai < aend && !(kb < ka));
-                k[i] = takeA ? ka : kb;
-                src[i] = takeA ? ai : bi;
-                ai += takeA ? 1u : 0u;
-                bi += takeA ? 0u : 1u;
-                const U nk = sk[pidx<ITEMS>(takeA ? ai : bi)];
-                ka = takeA ? nk : ka;
-                kb = takeA ? kb : nk;
+            for (int i = 0; i < PER; i++) {
+#pragma unroll
+                for (int h = 0; h < CH; h++) {
+                    const bool takeA = (bi[h] >= bend[h]) || (ai[h] < aend[h] && !(kb[h] < ka[h]));
+                    k[h * PER + i] = takeA ? ka[h] : kb[h];
+                    src[h * PER + i] = takeA ? ai[h] : bi[h];
+                    ai[h] += takeA ? 1u : 0u;
+                    bi[h] += takeA ? 0u : 1u;
+                    const U nk = sk[pidx<ITEMS>(takeA ? ai[h] : bi[h])];
+                    ka[h] = takeA ? nk : ka[h];
+                    kb[h] = takeA ? kb[h] : nk;
+                }
             }
             if (IDX) {
 #pragma unroll
@@ -260,6 +282,8 @@ __host__ __device__ constexpr uint32_t next_pow2(uint32_t x)
     while (p < x) p <<= 1;
     return p;
 }
+
+__host__ __device__ constexpr int ilog2(uint32_t x) { return x <= 1 ? 0 : 1 + ilog2(x >> 1); }
 
 __device__ __forceinline__ uint32_t next_pow2_dev(uint32_t x)
 {

@@ -1,6 +1,6 @@
 // ts_engine.h -- host-visible definitions of the sort engine: the device
 // control block, per-level work lists, segment records and the launch
-// parameters shared by ts_levels.hip and the C ABI.
+// parameters shared by the engine kernels and the C ABI.
 #pragma once
 #include <stdint.h>
 
@@ -21,11 +21,15 @@ enum : uint32_t { T_PIVOT = 1, T_COUNT = 2, T_SCATTER = 3, T_LDS = 4 };
 
 // kernel kinds of one engine run (launch timing index; the C ABI's
 // turtle_kernel_kind has the same values)
-enum : int { K_RESET = 0, K_ROOTS, K_LEVEL, K_PIVOT, K_COUNT, K_SCAN, K_SCATTER, K_CHILDREN, K_LDS, K_KINDS };
+enum : int {
+    K_RESET = 0, K_ROOTS, K_LEVEL, K_PIVOT, K_COUNT, K_SCAN, K_SCATTER, K_CHILDREN, K_LDS, K_LDSPREP, K_KINDS
+};
 
-constexpr int MAX_SPLIT = 15;
+constexpr int MAX_SPLIT = 31;
 constexpr int MAX_BINS = 2 * MAX_SPLIT + 1;
 constexpr int MAX_LEVELS = 64;
+constexpr int LDS_CLASSES = 16;        // LDS-sort size classes: n_eff = 2^c
+constexpr uint32_t SCAN_CHUNK = 4096;  // tile-count entries per workgroup of the level scan
 
 // Device control block (one per handle), cleared by the reset kernel before
 // every run.
@@ -41,6 +45,12 @@ struct Ctrl {
     unsigned long long bytes_scatter;
     unsigned long long bytes_lds;
     unsigned long long tasks[5];   // indexed by task kind
+    // LDS-sort tasks by size class (filled after the last level)
+    uint32_t cls_cnt[LDS_CLASSES];      // tasks per class
+    uint32_t cls_off[LDS_CLASSES];      // first slot of the class in the sorted task list
+    uint32_t cls_cur[LDS_CLASSES];      // fill cursor per class
+    uint32_t cls_wg[LDS_CLASSES + 1];   // first workgroup step of the class; [LDS_CLASSES] = total
+    uint32_t pad2[3];
 };
 
 // Segments of level L are the ids [sb, se) of the segment table.
@@ -48,18 +58,21 @@ struct Level {
     uint32_t sb, se, tiles, pad;
 };
 
-// A segment at or below the LDS cutoff, sorted in one workgroup.
+// A segment at or below the LDS cutoff, sorted in LDS.
 struct LdsTask {
     uint32_t begin, len, depth, pad;
 };
 
-// A segment being partitioned (length above the LDS cutoff).
+// A segment being partitioned (length above the LDS cutoff).  Its tiles are
+// [tile_base, tile_base + ntiles) of the level's tile table; their bin
+// counts sit at tcnt[tile_base*NB + bin*ntiles + tile] (bin-major), so the
+// level-wide exclusive scan of tcnt gives, relative to the segment's first
+// entry, the destination of every (bin, tile) inside the segment.
 struct Seg {
     uint32_t begin, len, depth, ntiles;
     uint32_t tile_base, pad0, pad1, pad2;
     uint64_t split[MAX_SPLIT];     // splitters in order representation, ascending
-    uint32_t base[MAX_BINS + 1];   // first position of each bin inside the segment
-    uint32_t total[MAX_BINS + 1];  // elements per bin
+    uint64_t pad3;
 };
 
 struct Params {
@@ -71,9 +84,12 @@ struct Params {
     uint32_t *it0, *it1;      // ping-pong indices
     Ctrl *ctrl;
     Seg *segs;
-    uint32_t *tiles;          // per-tile bin counts, then per-tile bin prefixes
+    uint32_t *tcnt;           // per-tile bin counts (bin-major per segment)
+    uint32_t *tpre;           // level-wide exclusive prefix of tcnt
+    uint32_t *spart;          // per-chunk sums of the level scan
     uint32_t *tile_seg;       // segment id of every tile of the current level
     LdsTask *ldst;
+    uint32_t *ldsq;           // LDS tasks ordered by size class
     Level *levels;
     uint32_t segcap, tilecap, ldscap, pad0;
     uint32_t cutoff;          // LDS sort at or below this length

@@ -1,0 +1,848 @@
+// ts_levels_impl.h -- the sort engine for gfx950: level-synchronous
+// partition passes over a device-resident work list, then LDS sorts.
+// Included once per key width by ts_levels_kb{1,2,4,8}.hip, which
+// instantiate it; ts_levels.hip dispatches.
+//
+// The reference recurses with CUDA dynamic parallelism: each wave of
+// quicksort blocks tail-launches the next wave (turtle-sort.cu:35-111,
+// 586-587) and one 1024-thread block partitions one whole segment
+// (turtle-sort.cu:573-934).  gfx950 has no device-side launch.  Here the
+// recursion runs level by level; one level is a short chain of kernels over
+// the segments of that level, and every segment is cut into tiles that all
+// workgroups share:
+//
+//   k_level    opens level L: its segments are ids [sb, se) of the segment
+//              table (created by the previous level), its tile count is 0;
+//   k_pivot    per segment: sample 1024 keys evenly, sort them in LDS, take
+//              S splitters at the sample quantiles (S = 3 is the reference's
+//              3-pivot scheme; 7, 15 and 31 cut a segment into more bins per
+//              pass), reserve the segment's tiles;
+//   k_count    per tile: bin counts (keys only);
+//   k_scan1/2  level-wide exclusive scan of the bin-major tile counts: the
+//              destination of every (segment, bin, tile) run;
+//   k_scatter  per tile: stable wave64 ranking into bins, staged in LDS,
+//              written out coalesced; pivot-equal bins go straight to the
+//              output (their keys are final), the other bins to the next
+//              ping-pong buffer;
+//   k_children per segment (one wave): every non-empty non-pivot bin becomes
+//              a segment of level L+1 or, at or below the cutoff, an LDS task.
+// After the last level the LDS tasks are grouped by size class
+// (k_ldscount, k_ldsplan, k_ldsbucket) and k_lds sorts them, 2^CMAX / 2^c
+// tasks of class c per workgroup step.
+//
+// Bins follow the reference's order LL,=pL,L,=p,R,=pR,RR
+// (turtle-sort.cu:755-761) generalised to S splitters: bin 2j+1 holds keys
+// equal to splitter j (equality tested first, lowest splitter first, as
+// turtle-sort.cu:688-699), bin 2j the keys strictly between splitters j-1
+// and j.
+//
+// Workgroups of one kernel never wait for each other: whatever one kernel
+// produces (counts, prefixes, segment records, list lengths) is read by a
+// later kernel on the same stream.  Cross-workgroup writes inside a kernel
+// are atomic appends to lists that only later kernels read.
+#pragma once
+#include "ts_device.h"
+#include "ts_engine.h"
+#include "ts_internal.h"
+
+namespace ts {
+
+template <int KB, bool IDX, int S>
+struct Cfg {
+    typedef typename KeyT<KB>::U U;
+    static constexpr int NT = 512;
+    static constexpr int NW = NT / WAVE;
+    static constexpr int NB = 2 * S + 1;
+    static constexpr int NBP = (int)next_pow2(NB + 1);   // bins + the "no element" marker NB
+    static constexpr int BITS = ilog2(NBP);               // bits of a bin id
+    static constexpr int LOG = ilog2(S + 1);              // depth of the splitter search tree
+    static constexpr int TI = (KB >= 8) ? 8 : 16;         // tile items per thread
+    static constexpr int SI = (KB >= 8) ? 8 : 16;         // LDS-sort items per thread
+    static constexpr uint32_t TILE = NT * TI;
+    static constexpr uint32_t CAP = NT * SI;
+    static constexpr uint32_t SAMPLE = 1024;
+    static constexpr uint32_t PP = CAP / SAMPLE;          // segments per pivot workgroup step
+    static constexpr uint32_t IB = IDX ? 4 : 0;
+    static constexpr int CMIN = ilog2(SI), CMAX = ilog2(CAP);
+    static constexpr uint32_t SORT_KEY_BYTES = (padded_len<SI>(CAP) * KB + 15) & ~15u;
+    static constexpr uint32_t SORT_BYTES = SORT_KEY_BYTES + (IDX ? padded_len<SI>(CAP) * 4 : 0);
+    static constexpr uint32_t SCAT_BYTES = TILE * KB + TILE * IB + TILE;
+    static_assert((S + 1) == (1 << LOG), "S + 1 must be a power of two");
+    static_assert(S <= MAX_SPLIT && NBP <= WAVE, "too many splitters");
+    static_assert(CMAX < LDS_CLASSES, "size classes");
+};
+
+// ---- buffers by depth -----------------------------------------------------------
+// depth 0 lives in kin; depth d >= 1 in kt[(d-1) & 1]; a depth-d segment
+// scatters its non-pivot bins into kt[d & 1].
+template <typename U>
+__device__ __forceinline__ const U *src_keys(const Params &p, uint32_t depth)
+{
+    return depth == 0 ? (const U *)p.kin : (const U *)((depth & 1) ? p.kt0 : p.kt1);
+}
+__device__ __forceinline__ const uint32_t *src_idx(const Params &p, uint32_t depth)
+{
+    return depth == 0 ? p.iin : ((depth & 1) ? p.it0 : p.it1);
+}
+template <typename U>
+__device__ __forceinline__ U *next_keys(const Params &p, uint32_t depth)
+{
+    return (U *)((depth & 1) ? p.kt1 : p.kt0);
+}
+__device__ __forceinline__ uint32_t *next_idx(const Params &p, uint32_t depth)
+{
+    return (depth & 1) ? p.it1 : p.it0;
+}
+
+// Records a failed range check (the caller skips the access); the run then
+// ends with ERR_RANGE instead of touching memory outside its buffers.
+__device__ __forceinline__ void range_fail(const Params &p, uint32_t where)
+{
+    if (atomic_or(&p.ctrl->error, ERR_RANGE) == 0) store_relaxed(&p.ctrl->bad_where, where);
+}
+
+__device__ __forceinline__ bool span_ok(const Params &p, uint32_t begin, uint32_t len)
+{
+    return len > 0 && begin <= p.n_total && len <= p.n_total - begin;
+}
+
+// appends n LDS-sort tasks, returns the first slot (or ~0u when full)
+__device__ __forceinline__ uint32_t lds_reserve(const Params &p, uint32_t n)
+{
+    const uint32_t b = atomic_add(&p.ctrl->lds_alloc, n);
+    if (b > p.ldscap || n > p.ldscap - b) {
+        atomic_or(&p.ctrl->error, ERR_LDSQ);
+        return ~0u;
+    }
+    return b;
+}
+
+// ---- splitters ----------------------------------------------------------------------
+// Sorted splitters in LDS, plus the same values in Eytzinger order (node i
+// has children 2i and 2i+1; tree[1] is the median) for a branch-free
+// search: after LOG steps i - (S+1) is the number of splitters < x.
+template <typename U, int S>
+struct Split {
+    U tree[S + 1];
+    U srt[S + 1];
+};
+
+template <typename U, int S, int LOG>
+__device__ __forceinline__ void split_load(Split<U, S> &sp, const Seg *sg)
+{
+    const uint32_t t = threadIdx.x;
+    if (t < (uint32_t)S) sp.srt[t] = (U)sg->split[t];
+    if (t >= 1 && t <= (uint32_t)S) {
+        const int d = 31 - __clz(t);                 // depth of node t
+        const uint32_t k = t - (1u << d);            // position inside its depth
+        const uint32_t r = (2 * k + 1) * (1u << (LOG - 1 - d)) - 1;   // in-order rank
+        sp.tree[t] = (U)sg->split[r];
+    }
+}
+
+// bin of order value x: lt = #splitters < x; equal to a splitter -> 2*lt+1
+// (lt then indexes the lowest equal splitter), else 2*lt
+template <typename U, int S, int LOG>
+__device__ __forceinline__ uint32_t classify(U x, const Split<U, S> &sp, const U (&reg)[S <= 7 ? S : 1])
+{
+    uint32_t lt;
+    bool eq;
+    if constexpr (S <= 7) {
+        uint32_t le = 0;
+        lt = 0;
+#pragma unroll
+        for (int j = 0; j < S; j++) {
+            lt += (reg[j] < x) ? 1u : 0u;
+            le += (reg[j] <= x) ? 1u : 0u;
+        }
+        eq = le > lt;
+    } else {
+        uint32_t i = 1;
+#pragma unroll
+        for (int l = 0; l < LOG; l++) i = 2 * i + ((sp.tree[i] < x) ? 1u : 0u);
+        lt = i - (S + 1);
+        eq = lt < (uint32_t)S && sp.srt[lt] == x;
+    }
+    return eq ? 2 * lt + 1 : 2 * lt;
+}
+
+// LDS-sort size class of a task of len keys: n_eff = 2^class
+// (clamped to the class table; a task never exceeds the LDS capacity)
+template <int CMIN>
+__device__ __forceinline__ uint32_t lds_class(uint32_t len)
+{
+    const uint32_t c = len <= 1 ? 0u : (uint32_t)(32 - __clz(len - 1));
+    return c < (uint32_t)CMIN ? (uint32_t)CMIN : (c < (uint32_t)LDS_CLASSES ? c : (uint32_t)LDS_CLASSES - 1);
+}
+
+// ---- set-up kernels ------------------------------------------------------------------
+__global__ void k_reset(Ctrl *c);
+__global__ void k_roots(Params p, const uint32_t *off, uint32_t nseg, uint32_t single_len);
+__global__ void k_level(Params p, uint32_t L);
+__global__ void k_ldsplan(Params p, uint32_t cmin, uint32_t cmax, uint32_t cap);
+
+template <int KB, bool IDX, int S>
+struct LevelK {
+    typedef Cfg<KB, IDX, S> C;
+    typedef typename C::U U;
+    static constexpr int NT = C::NT, NW = C::NW, NB = C::NB, NBP = C::NBP, BITS = C::BITS, LOG = C::LOG;
+    static constexpr int TI = C::TI, SI = C::SI;
+    static constexpr uint32_t TILE = C::TILE, CAP = C::CAP;
+    typedef U Reg[S <= 7 ? S : 1];
+
+    struct SharedScan {
+        uint32_t red[16];
+        uint32_t bcast[4];
+    };
+    struct SharedTile {
+        uint32_t wcnt[NW][NBP];   // per-wave bin counts
+        uint32_t woff[NW][NBP];   // per-wave bin offsets inside the tile
+        uint32_t dbase[NBP];      // destination of bin b = dbase[b] + staging position
+        Split<U, S> sp;
+    };
+    struct TaskRec {
+        uint32_t begin, len, depth;
+    };
+
+    static __device__ __forceinline__ void load_reg(const Split<U, S> &sp, Reg &reg)
+    {
+        if constexpr (S <= 7) {
+#pragma unroll
+            for (int j = 0; j < S; j++) reg[j] = sp.srt[j];
+        }
+    }
+
+    // segment id + tile of global tile g, validated
+    static __device__ __forceinline__ bool tile_of(const Params &p, uint32_t g, uint32_t &sid, uint32_t &t)
+    {
+        sid = p.tile_seg[g];
+        if (sid >= p.segcap) return false;
+        const Seg *sg = &p.segs[sid];
+        if (g < sg->tile_base) return false;
+        t = g - sg->tile_base;
+        return t < sg->ntiles && span_ok(p, sg->begin, sg->len) && (uint64_t)t * TILE < sg->len;
+    }
+
+    // ---------------------------------------------------------------- pivot
+    // PP segments per workgroup step: their 1024-key samples are sorted as
+    // independent runs of one block sort.
+    static __device__ void pivot(const Params &p, uint32_t L, unsigned char *arena, TaskRec *rec, SharedScan &sh)
+    {
+        const Level lv = p.levels[L];
+        U *sk = (U *)arena;
+        const U flip = (U)p.flip;
+        constexpr uint32_t PP = C::PP, SAMPLE = C::SAMPLE;
+        for (uint32_t s0 = lv.sb + blockIdx.x * PP; s0 < lv.se; s0 += gridDim.x * PP) {
+            const uint32_t nh = (lv.se - s0) < PP ? (lv.se - s0) : PP;
+            if (threadIdx.x < nh) {
+                const Seg *sg = &p.segs[s0 + threadIdx.x];
+                TaskRec r{sg->begin, sg->len, sg->depth};
+                if (!span_ok(p, r.begin, r.len)) {
+                    range_fail(p, 3);
+                    r.len = 0;
+                }
+                rec[threadIdx.x] = r;
+            }
+            block_sync();
+            for (uint32_t i = threadIdx.x; i < CAP; i += NT) {
+                const uint32_t q = i / SAMPLE, j = i % SAMPLE;
+                U v = key_max<U>();
+                if (q < nh) {
+                    const TaskRec r = rec[q];
+                    const uint32_t m = r.len < SAMPLE ? r.len : SAMPLE;
+                    if (j < m) {
+                        const uint64_t pos = ((2ull * j + 1ull) * (uint64_t)r.len) / (2ull * m);
+                        v = (U)(src_keys<U>(p, r.depth)[r.begin + (uint32_t)pos] ^ flip);
+                    }
+                }
+                sk[pidx<SI>(i)] = v;
+            }
+            block_sync();
+            block_sort<U, false, NT, SI>(sk, nullptr, CAP, SAMPLE);
+            if (threadIdx.x < nh) {
+                const uint32_t q = threadIdx.x;
+                const TaskRec r = rec[q];
+                Seg *sg = &p.segs[s0 + q];
+                uint32_t tb = ~0u, ntiles = 0;
+                if (r.len > 0) {
+                    const uint32_t m = r.len < SAMPLE ? r.len : SAMPLE;
+                    for (int j = 0; j < S; j++)
+                        sg->split[j] = (uint64_t)sk[pidx<SI>(q * SAMPLE + ((uint32_t)(j + 1) * m) / (S + 1))];
+                    ntiles = (r.len + TILE - 1) / TILE;
+                    tb = atomic_add(&p.ctrl->tile_alloc, ntiles);
+                    if (tb > p.tilecap || ntiles > p.tilecap - tb) {
+                        atomic_or(&p.ctrl->error, ERR_TILES);
+                        tb = ~0u;
+                    }
+                }
+                sg->ntiles = tb == ~0u ? 0u : ntiles;
+                sg->tile_base = tb == ~0u ? 0u : tb;
+                rec[q].len = tb == ~0u ? 0u : ntiles;   // reused: tiles to map
+                rec[q].begin = tb;
+            }
+            if (threadIdx.x == 0) atomic_add64((unsigned long long *)&p.ctrl->tasks[T_PIVOT], (unsigned long long)nh);
+            block_sync();
+            for (uint32_t q = 0; q < nh; q++) {
+                const uint32_t tb = rec[q].begin, nt = rec[q].len;
+                for (uint32_t t = threadIdx.x; t < nt; t += NT) p.tile_seg[tb + t] = s0 + q;
+            }
+            block_sync();
+        }
+    }
+
+    // ---------------------------------------------------------------- count
+    static __device__ void count(const Params &p, SharedTile &sh)
+    {
+        const uint32_t ntot = p.ctrl->tile_alloc < p.tilecap ? p.ctrl->tile_alloc : p.tilecap;
+        const uint32_t G = p.tiles_per_wg;
+        const U flip = (U)p.flip;
+        const int w = threadIdx.x / WAVE;
+        uint64_t nbytes = 0;
+        uint32_t cur = ~0u;
+        Reg reg;
+        for (uint32_t c0 = blockIdx.x * G; c0 < ntot; c0 += gridDim.x * G) {
+            const uint32_t c1 = c0 + G < ntot ? c0 + G : ntot;
+            for (uint32_t g = c0; g < c1; g++) {
+                uint32_t sid, t;
+                if (!tile_of(p, g, sid, t)) {
+                    if (threadIdx.x == 0) range_fail(p, 4);
+                    continue;
+                }
+                const Seg *sg = &p.segs[sid];
+                if (sid != cur) {
+                    split_load<U, S, LOG>(sh.sp, sg);
+                    block_sync();
+                    cur = sid;
+                }
+                load_reg(sh.sp, reg);
+                const U *src = src_keys<U>(p, sg->depth) + sg->begin;
+                const uint32_t ts = t * TILE;
+                const uint32_t tl = (sg->len - ts) < TILE ? (sg->len - ts) : TILE;
+                nbytes += (uint64_t)tl * KB;
+                if (lane() < (uint32_t)NBP) sh.wcnt[w][lane()] = 0;
+                wave_sync();
+                U x[TI];
+#pragma unroll
+                for (int c = 0; c < TI; c++) {
+                    const uint32_t e = (uint32_t)w * (TI * WAVE) + c * WAVE + lane();
+                    x[c] = e < tl ? src[ts + e] : (U)0;
+                }
+                // all searches first (loads only, so they overlap), then the
+                // counter updates: one LDS atomic per key, conflicts between
+                // lanes of equal bins are serialised by the LDS
+                uint32_t bins[TI];
+#pragma unroll
+                for (int c = 0; c < TI; c++) bins[c] = classify<U, S, LOG>((U)(x[c] ^ flip), sh.sp, reg);
+#pragma unroll
+                for (int c = 0; c < TI; c++) {
+                    const uint32_t e = (uint32_t)w * (TI * WAVE) + c * WAVE + lane();
+                    if (e < tl) atomicAdd(&sh.wcnt[w][bins[c]], 1u);
+                }
+                block_sync();
+                if (threadIdx.x < (uint32_t)NB) {
+                    uint32_t s = 0;
+#pragma unroll
+                    for (int i = 0; i < NW; i++) s += sh.wcnt[i][threadIdx.x];
+                    p.tcnt[(size_t)sg->tile_base * NB + (size_t)threadIdx.x * sg->ntiles + t] = s;
+                }
+                block_sync();
+            }
+        }
+        if (threadIdx.x == 0 && nbytes) {
+            atomic_add64(&p.ctrl->bytes_count, nbytes);
+            atomic_add64(&p.ctrl->tasks[T_COUNT], 1ull);
+        }
+    }
+
+    // ---------------------------------------------------------------- scan
+    // Level-wide exclusive scan of tcnt[0, tiles*NB) in chunks of SCAN_CHUNK
+    // entries: scan1 sums every chunk, scan2 adds the sums of the chunks
+    // before it and scans inside.  Values are < 2^31 (keys of one level).
+    static __device__ void scan1(const Params &p, SharedScan &sh)
+    {
+        const uint32_t nt = p.ctrl->tile_alloc < p.tilecap ? p.ctrl->tile_alloc : p.tilecap;
+        const uint32_t E = nt * (uint32_t)NB;
+        const uint32_t nch = (E + SCAN_CHUNK - 1) / SCAN_CHUNK;
+        for (uint32_t ch = blockIdx.x; ch < nch; ch += gridDim.x) {
+            const uint32_t a = ch * SCAN_CHUNK, b = (a + SCAN_CHUNK) < E ? a + SCAN_CHUNK : E;
+            uint32_t s = 0;
+            for (uint32_t i = a + threadIdx.x; i < b; i += NT) s += p.tcnt[i];
+            uint32_t tot;
+            (void)block_excl_scan<NT>(s, sh.red, &tot);
+            if (threadIdx.x == 0) p.spart[ch] = tot;
+        }
+    }
+
+    static __device__ void scan2(const Params &p, SharedScan &sh)
+    {
+        constexpr uint32_t PER = SCAN_CHUNK / NT;
+        const uint32_t nt = p.ctrl->tile_alloc < p.tilecap ? p.ctrl->tile_alloc : p.tilecap;
+        const uint32_t E = nt * (uint32_t)NB;
+        const uint32_t nch = (E + SCAN_CHUNK - 1) / SCAN_CHUNK;
+        for (uint32_t ch = blockIdx.x; ch < nch; ch += gridDim.x) {
+            uint32_t s = 0;
+            for (uint32_t c = threadIdx.x; c < ch; c += NT) s += p.spart[c];
+            uint32_t off;
+            (void)block_excl_scan<NT>(s, sh.red, &off);
+            const uint32_t a = ch * SCAN_CHUNK + threadIdx.x * PER;
+            uint32_t v[PER];
+            uint32_t loc = 0;
+#pragma unroll
+            for (int k = 0; k < (int)PER; k++) {
+                v[k] = (a + k < E) ? p.tcnt[a + k] : 0u;
+                loc += v[k];
+            }
+            uint32_t tot;
+            uint32_t run = off + block_excl_scan<NT>(loc, sh.red, &tot);
+#pragma unroll
+            for (int k = 0; k < (int)PER; k++) {
+                if (a + k < E) p.tpre[a + k] = run;
+                run += v[k];
+            }
+        }
+    }
+
+    // ---------------------------------------------------------------- scatter
+    static __device__ void scatter(const Params &p, unsigned char *arena, SharedTile &sh)
+    {
+        const uint32_t ntot = p.ctrl->tile_alloc < p.tilecap ? p.ctrl->tile_alloc : p.tilecap;
+        const uint32_t G = p.tiles_per_wg;
+        const U flip = (U)p.flip;
+        U *stk = (U *)arena;
+        uint32_t *sti = (uint32_t *)(arena + TILE * KB);
+        unsigned char *stb = arena + TILE * KB + TILE * C::IB;
+        U *kfin = (U *)p.kout;
+        uint32_t *ifin = p.iout;
+        const int w = threadIdx.x / WAVE;
+        uint64_t nbytes = 0;
+        uint32_t cur = ~0u;
+        Reg reg;
+        for (uint32_t c0 = blockIdx.x * G; c0 < ntot; c0 += gridDim.x * G) {
+            const uint32_t c1 = c0 + G < ntot ? c0 + G : ntot;
+            for (uint32_t g = c0; g < c1; g++) {
+                uint32_t sid, t;
+                if (!tile_of(p, g, sid, t)) {
+                    if (threadIdx.x == 0) range_fail(p, 6);
+                    continue;
+                }
+                const Seg *sg = &p.segs[sid];
+                if (sid != cur) {
+                    split_load<U, S, LOG>(sh.sp, sg);
+                    block_sync();
+                    cur = sid;
+                }
+                load_reg(sh.sp, reg);
+                const uint32_t begin = sg->begin, len = sg->len, depth = sg->depth;
+                const uint32_t tb = sg->tile_base, ntl = sg->ntiles;
+                const U *src = src_keys<U>(p, depth) + begin;
+                const uint32_t *isrc = src_idx(p, depth) + begin;
+                U *knext = next_keys<U>(p, depth);
+                uint32_t *inext = next_idx(p, depth);
+                const uint32_t ts = t * TILE;
+                const uint32_t tl = (len - ts) < TILE ? (len - ts) : TILE;
+                nbytes += 2ull * tl * (KB + C::IB);
+                if (lane() < (uint32_t)NBP) sh.wcnt[w][lane()] = 0;
+                wave_sync();
+                U x[TI];
+                uint32_t ix[TI];
+                uint32_t pk[TI];   // bin | rank inside the wave's bin << 8
+#pragma unroll
+                for (int c = 0; c < TI; c++) {
+                    const uint32_t e = (uint32_t)w * (TI * WAVE) + c * WAVE + lane();
+                    x[c] = e < tl ? src[ts + e] : (U)0;
+                    if (IDX) ix[c] = e < tl ? isrc[ts + e] : 0u;
+                }
+                // rank: running per-wave bin counter (one wave owns wcnt[w],
+                // its LDS accesses complete in program order) + rank among
+                // the lanes of this step with the same bin
+#pragma unroll
+                for (int c = 0; c < TI; c++) pk[c] = classify<U, S, LOG>((U)(x[c] ^ flip), sh.sp, reg);
+#pragma unroll
+                for (int c = 0; c < TI; c++) {
+                    const uint32_t e = (uint32_t)w * (TI * WAVE) + c * WAVE + lane();
+                    const bool valid = e < tl;
+                    const uint32_t bin = valid ? pk[c] : (uint32_t)NB;
+                    const uint64_t peers = match_peers<BITS>(bin, valid);
+                    const uint32_t r = mask_rank(peers);
+                    const uint32_t old = sh.wcnt[w][bin];
+                    if (valid && r == 0) sh.wcnt[w][bin] = old + (uint32_t)__popcll(peers);
+                    pk[c] = bin | ((old + r) << 8);
+                }
+                block_sync();
+                if (w == 0) {
+                    const uint32_t l = lane();
+                    uint32_t tot = 0;
+                    if (l < (uint32_t)NB) {
+#pragma unroll
+                        for (int i = 0; i < NW; i++) tot += sh.wcnt[i][l];
+                    }
+                    const uint32_t start = wave_incl_scan(tot) - tot;
+                    if (l < (uint32_t)NB) {
+                        uint32_t run = start;
+#pragma unroll
+                        for (int i = 0; i < NW; i++) {
+                            sh.woff[i][l] = run;
+                            run += sh.wcnt[i][l];
+                        }
+                        const size_t e0 = (size_t)tb * NB;
+                        const uint32_t rel = p.tpre[e0 + (size_t)l * ntl + t] - p.tpre[e0];
+                        sh.dbase[l] = begin + rel - start;
+                    }
+                }
+                block_sync();
+#pragma unroll
+                for (int c = 0; c < TI; c++) {
+                    const uint32_t e = (uint32_t)w * (TI * WAVE) + c * WAVE + lane();
+                    if (e < tl) {
+                        const uint32_t bin = pk[c] & 0xffu;
+                        const uint32_t pos = sh.woff[w][bin] + (pk[c] >> 8);
+                        if (pos < tl) {
+                            stk[pos] = x[c];
+                            if (IDX) sti[pos] = ix[c];
+                            stb[pos] = (unsigned char)bin;
+                        } else {
+                            range_fail(p, 8);
+                        }
+                    }
+                }
+                block_sync();
+                for (uint32_t i = threadIdx.x; i < tl; i += NT) {
+                    const uint32_t b = stb[i];
+                    const uint32_t d = b < (uint32_t)NB ? sh.dbase[b] + i : 0xffffffffu;
+                    if (d - begin >= len) {        // outside the segment (also catches b >= NB)
+                        range_fail(p, 9);
+                        continue;
+                    }
+                    U *kd = (b & 1u) ? kfin : knext;
+                    kd[d] = stk[i];
+                    if (IDX) {
+                        uint32_t *id = (b & 1u) ? ifin : inext;
+                        id[d] = sti[i];
+                    }
+                }
+                block_sync();
+            }
+        }
+        if (threadIdx.x == 0 && nbytes) {
+            atomic_add64(&p.ctrl->bytes_scatter, nbytes);
+            atomic_add64(&p.ctrl->tasks[T_SCATTER], 1ull);
+        }
+    }
+
+    // ---------------------------------------------------------------- children
+    // one wave per segment of level L; lane j handles the non-pivot bin 2j
+    static __device__ void children(const Params &p, uint32_t L)
+    {
+        const Level lv = p.levels[L];
+        const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+        const uint32_t nwv = gridDim.x * blockDim.x / WAVE;
+        for (uint32_t sid = lv.sb + wid; sid < lv.se; sid += nwv) {
+            const Seg *sg = &p.segs[sid];
+            const uint32_t nt = sg->ntiles;
+            if (nt == 0) continue;   // overflowed segment
+            const uint32_t begin = sg->begin, len = sg->len, depth = sg->depth;
+            const size_t e0 = (size_t)sg->tile_base * NB, e1 = e0 + (size_t)NB * nt;
+            const uint32_t p0 = p.tpre[e0];
+            const uint32_t sum = p.tpre[e1 - 1] + p.tcnt[e1 - 1] - p0;
+            if (sum != len) {
+                if (lane() == 0) atomic_or(&p.ctrl->error, ERR_BAD);
+                continue;
+            }
+            const uint32_t j = lane();
+            uint32_t cl = 0, cb = 0;
+            if (j <= (uint32_t)S) {
+                const uint32_t b = 2 * j;
+                const uint32_t st = p.tpre[e0 + (size_t)b * nt] - p0;
+                const uint32_t en = (b + 1 < (uint32_t)NB) ? p.tpre[e0 + (size_t)(b + 1) * nt] - p0 : len;
+                cl = en - st;
+                cb = begin + st;
+            }
+            const bool big = cl > p.cutoff;
+            const bool small = cl > 0 && !big;
+            const uint64_t mb = ballot(big), ms = ballot(small);
+            uint32_t sbase = 0, lbase = 0;
+            if (lane() == 0) {
+                const uint32_t nb = (uint32_t)__popcll(mb), ns = (uint32_t)__popcll(ms);
+                if (nb) sbase = atomic_add(&p.ctrl->seg_alloc, nb);
+                if (ns) lbase = lds_reserve(p, ns);
+            }
+            sbase = __shfl(sbase, 0, WAVE);
+            lbase = __shfl(lbase, 0, WAVE);
+            if (big) {
+                const uint32_t cid = sbase + mask_rank(mb);
+                if (cid >= p.segcap) {
+                    atomic_or(&p.ctrl->error, ERR_SEGS);
+                } else {
+                    Seg *c = &p.segs[cid];
+                    c->begin = cb;
+                    c->len = cl;
+                    c->depth = depth + 1;
+                }
+            } else if (small && lbase != ~0u) {
+                LdsTask *tk = &p.ldst[lbase + mask_rank(ms)];
+                tk->begin = cb;
+                tk->len = cl;
+                tk->depth = depth + 1;
+            }
+        }
+    }
+
+    // ---------------------------------------------------------------- LDS sort
+    // tasks per size class (per-workgroup LDS histogram, one atomic per class)
+    static __device__ void ldscount(const Params &p, uint32_t *hist)
+    {
+        const uint32_t ntask = p.ctrl->lds_alloc < p.ldscap ? p.ctrl->lds_alloc : p.ldscap;
+        if (threadIdx.x < (uint32_t)LDS_CLASSES) hist[threadIdx.x] = 0;
+        block_sync();
+        for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < ntask; k += gridDim.x * blockDim.x)
+            atomicAdd(&hist[lds_class<C::CMIN>(p.ldst[k].len)], 1u);
+        block_sync();
+        if (threadIdx.x < (uint32_t)LDS_CLASSES && hist[threadIdx.x]) atomic_add(&p.ctrl->cls_cnt[threadIdx.x], hist[threadIdx.x]);
+    }
+
+    // task ids ordered by class (order inside a class is arbitrary; every
+    // task is sorted on its own, so the output does not depend on it)
+    static __device__ void ldsbucket(const Params &p, uint32_t *hist, uint32_t *wbase)
+    {
+        const uint32_t ntask = p.ctrl->lds_alloc < p.ldscap ? p.ctrl->lds_alloc : p.ldscap;
+        const uint32_t stride = gridDim.x * blockDim.x;
+        for (uint32_t k0 = blockIdx.x * blockDim.x; k0 < ntask; k0 += stride) {
+            if (threadIdx.x < (uint32_t)LDS_CLASSES) hist[threadIdx.x] = 0;
+            block_sync();
+            const uint32_t k = k0 + threadIdx.x;
+            uint32_t c = 0, lr = 0;
+            if (k < ntask) {
+                c = lds_class<C::CMIN>(p.ldst[k].len);
+                lr = atomicAdd(&hist[c], 1u);
+            }
+            block_sync();
+            if (threadIdx.x < (uint32_t)LDS_CLASSES && hist[threadIdx.x])
+                wbase[threadIdx.x] = atomic_add(&p.ctrl->cls_cur[threadIdx.x], hist[threadIdx.x]);
+            block_sync();
+            if (k < ntask) {
+                const uint32_t slot = p.ctrl->cls_off[c] + wbase[c] + lr;
+                if (slot < p.ldscap) p.ldsq[slot] = k;
+                else range_fail(p, 11);
+            }
+            block_sync();
+        }
+    }
+
+    static __device__ void lds(const Params &p, unsigned char *arena, TaskRec *rec, unsigned long long *acc)
+    {
+        U *sk = (U *)arena;
+        uint32_t *si = (uint32_t *)(arena + C::SORT_KEY_BYTES);
+        const U flip = (U)p.flip;
+        U *dst = (U *)p.kout;
+        const Ctrl *cc = p.ctrl;
+        const uint32_t total = cc->cls_wg[LDS_CLASSES];
+        uint64_t nbytes = 0, ntasks = 0;
+        if (threadIdx.x == 0) acc[0] = acc[1] = 0;
+        for (uint32_t v = blockIdx.x; v < total; v += gridDim.x) {
+            uint32_t c = C::CMIN;
+            while (c < (uint32_t)C::CMAX && v >= cc->cls_wg[c + 1]) c++;
+            const uint32_t n_eff = 1u << c, P = CAP >> c;
+            const uint32_t first = (v - cc->cls_wg[c]) * P;
+            const uint32_t cnt = cc->cls_cnt[c];
+            const uint32_t nk = first < cnt ? ((cnt - first) < P ? (cnt - first) : P) : 0u;
+            for (uint32_t q = threadIdx.x; q < P; q += NT) {
+                TaskRec r{0, 0, 0};
+                if (q < nk) {
+                    const uint32_t slot = cc->cls_off[c] + first + q;
+                    const uint32_t k = slot < p.ldscap ? p.ldsq[slot] : ~0u;
+                    if (k < p.ldscap) {
+                        const LdsTask tk = p.ldst[k];
+                        r = TaskRec{tk.begin, tk.len, tk.depth};
+                    }
+                    if (!span_ok(p, r.begin, r.len) || r.len > n_eff) {
+                        range_fail(p, 1);
+                        r.len = 0;
+                    }
+                    nbytes += 2ull * r.len * (KB + C::IB);
+                    ntasks += r.len ? 1u : 0u;
+                }
+                rec[q] = r;
+            }
+            block_sync();
+            for (uint32_t i = threadIdx.x; i < CAP; i += NT) {
+                const uint32_t q = i >> c, j = i & (n_eff - 1);
+                const TaskRec r = rec[q];
+                const bool on = j < r.len;
+                sk[pidx<SI>(i)] = on ? (U)(src_keys<U>(p, r.depth)[r.begin + j] ^ flip) : key_max<U>();
+                if (IDX) si[pidx<SI>(i)] = on ? src_idx(p, r.depth)[r.begin + j] : 0u;
+            }
+            block_sync();
+            block_sort<U, IDX, NT, SI>(sk, si, CAP, n_eff);
+            for (uint32_t i = threadIdx.x; i < CAP; i += NT) {
+                const uint32_t q = i >> c, j = i & (n_eff - 1);
+                const TaskRec r = rec[q];
+                if (j < r.len) {
+                    dst[r.begin + j] = (U)(sk[pidx<SI>(i)] ^ flip);
+                    if (IDX) p.iout[r.begin + j] = si[pidx<SI>(i)];
+                }
+            }
+            block_sync();
+        }
+        if (ntasks) {
+            atomicAdd(&acc[0], (unsigned long long)nbytes);
+            atomicAdd(&acc[1], (unsigned long long)ntasks);
+        }
+        block_sync();
+        if (threadIdx.x == 0 && acc[1]) {
+            atomic_add64(&p.ctrl->bytes_lds, acc[0]);
+            atomic_add64(&p.ctrl->tasks[T_LDS], acc[1]);
+        }
+    }
+};
+
+// ---- kernels ----------------------------------------------------------------------------
+template <int KB, bool IDX, int S>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k_pivot(Params p, uint32_t L)
+{
+    typedef LevelK<KB, IDX, S> K;
+    __shared__ __align__(16) unsigned char arena[Cfg<KB, IDX, S>::SORT_KEY_BYTES];
+    __shared__ typename K::TaskRec rec[Cfg<KB, IDX, S>::PP];
+    __shared__ typename K::SharedScan sh;
+    K::pivot(p, L, arena, rec, sh);
+}
+
+template <int KB, bool IDX, int S>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k_count(Params p)
+{
+    typedef LevelK<KB, IDX, S> K;
+    __shared__ typename K::SharedTile sh;
+    K::count(p, sh);
+}
+
+template <int KB, bool IDX, int S>
+__global__ void __launch_bounds__(512) k_scan1(Params p)
+{
+    typedef LevelK<KB, IDX, S> K;
+    __shared__ typename K::SharedScan sh;
+    K::scan1(p, sh);
+}
+
+template <int KB, bool IDX, int S>
+__global__ void __launch_bounds__(512) k_scan2(Params p)
+{
+    typedef LevelK<KB, IDX, S> K;
+    __shared__ typename K::SharedScan sh;
+    K::scan2(p, sh);
+}
+
+template <int KB, bool IDX, int S>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k_scatter(Params p)
+{
+    typedef LevelK<KB, IDX, S> K;
+    __shared__ __align__(16) unsigned char arena[Cfg<KB, IDX, S>::SCAT_BYTES];
+    __shared__ typename K::SharedTile sh;
+    K::scatter(p, arena, sh);
+}
+
+template <int KB, bool IDX, int S>
+__global__ void __launch_bounds__(256) k_children(Params p, uint32_t L)
+{
+    LevelK<KB, IDX, S>::children(p, L);
+}
+
+template <int KB, bool IDX, int S>
+__global__ void __launch_bounds__(256) k_ldscount(Params p)
+{
+    __shared__ uint32_t hist[LDS_CLASSES];
+    LevelK<KB, IDX, S>::ldscount(p, hist);
+}
+
+template <int KB, bool IDX, int S>
+__global__ void __launch_bounds__(256) k_ldsbucket(Params p)
+{
+    __shared__ uint32_t hist[LDS_CLASSES];
+    __shared__ uint32_t wbase[LDS_CLASSES];
+    LevelK<KB, IDX, S>::ldsbucket(p, hist, wbase);
+}
+
+template <int KB, bool IDX, int S>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k_lds(Params p)
+{
+    typedef LevelK<KB, IDX, S> K;
+    __shared__ __align__(16) unsigned char arena[Cfg<KB, IDX, S>::SORT_BYTES];
+    __shared__ typename K::TaskRec rec[Cfg<KB, IDX, S>::CAP / Cfg<KB, IDX, S>::SI];
+    __shared__ unsigned long long acc[2];
+    K::lds(p, arena, rec, acc);
+}
+
+// ---- host side ------------------------------------------------------------------------
+template <int KB, bool IDX, int S>
+static int occ(const void *k, int threads)
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, threads, 0) != hipSuccess || nb < 1) return 1;
+    return nb;
+}
+
+template <int KB, bool IDX, int S>
+hipError_t run_levels(const Params &p, int cus, int wg_per_cu, const uint32_t *dev_off, uint32_t nseg,
+                      uint32_t single_len, uint32_t L0, uint32_t nlev, bool first, bool last, hipStream_t st,
+                      LaunchTimer *tm)
+{
+    typedef Cfg<KB, IDX, S> C;
+    static int o_tile = 0, o_cnt = 0, o_pivot = 0, o_lds = 0;
+    if (!o_tile) {
+        o_cnt = occ<KB, IDX, S>((const void *)k_count<KB, IDX, S>, 512);
+        o_pivot = occ<KB, IDX, S>((const void *)k_pivot<KB, IDX, S>, 512);
+        o_lds = occ<KB, IDX, S>((const void *)k_lds<KB, IDX, S>, 512);
+        o_tile = occ<KB, IDX, S>((const void *)k_scatter<KB, IDX, S>, 512);
+    }
+    auto lim = [&](int o) { return cus * ((wg_per_cu > 0 && wg_per_cu < o) ? wg_per_cu : o); };
+    const int g_tile = lim(o_tile), g_cnt = lim(o_cnt), g_pivot = cus * o_pivot, g_lds = cus * o_lds;
+    const int g_small = cus * 2;
+#define TS_LAUNCH(kind, ...)                  \
+    do {                                      \
+        if (tm) tm->begin(kind, st);          \
+        hipLaunchKernelGGL(__VA_ARGS__);      \
+        if (tm) tm->end(kind, st);            \
+    } while (0)
+    if (first) {
+        TS_LAUNCH(K_RESET, k_reset, dim3(1), dim3(256), 0, st, p.ctrl);
+        if (nseg > 0)
+            TS_LAUNCH(K_ROOTS, k_roots, dim3((nseg + 255) / 256), dim3(256), 0, st, p, dev_off, nseg, single_len);
+    }
+    for (uint32_t L = L0; L < L0 + nlev && L < (uint32_t)MAX_LEVELS; L++) {
+        TS_LAUNCH(K_LEVEL, k_level, dim3(1), dim3(64), 0, st, p, L);
+        TS_LAUNCH(K_PIVOT, (k_pivot<KB, IDX, S>), dim3(g_pivot), dim3(512), 0, st, p, L);
+        TS_LAUNCH(K_COUNT, (k_count<KB, IDX, S>), dim3(g_cnt), dim3(512), 0, st, p);
+        TS_LAUNCH(K_SCAN, (k_scan1<KB, IDX, S>), dim3(g_small), dim3(512), 0, st, p);
+        TS_LAUNCH(K_SCAN, (k_scan2<KB, IDX, S>), dim3(g_small), dim3(512), 0, st, p);
+        TS_LAUNCH(K_SCATTER, (k_scatter<KB, IDX, S>), dim3(g_tile), dim3(512), 0, st, p);
+        TS_LAUNCH(K_CHILDREN, (k_children<KB, IDX, S>), dim3(g_small), dim3(256), 0, st, p, L);
+    }
+    if (last) {
+        TS_LAUNCH(K_LDSPREP, (k_ldscount<KB, IDX, S>), dim3(g_small), dim3(256), 0, st, p);
+        TS_LAUNCH(K_LDSPREP, k_ldsplan, dim3(1), dim3(64), 0, st, p, (uint32_t)C::CMIN, (uint32_t)C::CMAX, C::CAP);
+        TS_LAUNCH(K_LDSPREP, (k_ldsbucket<KB, IDX, S>), dim3(g_small), dim3(256), 0, st, p);
+        TS_LAUNCH(K_LDS, (k_lds<KB, IDX, S>), dim3(g_lds), dim3(512), 0, st, p);
+    }
+#undef TS_LAUNCH
+    return hipGetLastError();
+}
+
+typedef hipError_t (*levels_fn)(const Params &, int, int, const uint32_t *, uint32_t, uint32_t, uint32_t, uint32_t,
+                                bool, bool, hipStream_t, LaunchTimer *);
+
+template <int KB>
+levels_fn pick_kb(bool idx, int S)
+{
+    switch (S) {
+    case 3: return idx ? run_levels<KB, true, 3> : run_levels<KB, false, 3>;
+    case 7: return idx ? run_levels<KB, true, 7> : run_levels<KB, false, 7>;
+    case 15: return idx ? run_levels<KB, true, 15> : run_levels<KB, false, 15>;
+    case 31: return idx ? run_levels<KB, true, 31> : run_levels<KB, false, 31>;
+    default: return nullptr;
+    }
+}
+
+levels_fn pick_kb1(bool idx, int S);
+levels_fn pick_kb2(bool idx, int S);
+levels_fn pick_kb4(bool idx, int S);
+levels_fn pick_kb8(bool idx, int S);
+
+}  // namespace ts

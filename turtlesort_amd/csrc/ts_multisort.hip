@@ -26,7 +26,7 @@ __global__ void __launch_bounds__(NT) ms_kernel(const U *__restrict__ in, U *__r
     for (uint32_t i = threadIdx.x; i < n_eff; i += NT)
         sk[pidx<ITEMS>(i)] = i < arr_size ? (U)(in[base + i] ^ flip) : key_max<U>();
     block_sync();
-    block_sort<U, false, NT, ITEMS>(sk, nullptr, n_eff);
+    block_sort<U, false, NT, ITEMS>(sk, nullptr, n_eff, n_eff);
     for (uint32_t i = threadIdx.x; i < arr_size; i += NT) out[base + i] = (U)(sk[pidx<ITEMS>(i)] ^ flip);
 }
 
