@@ -172,13 +172,17 @@ __device__ __forceinline__ void block_sort(U *sk, uint32_t *si, uint32_t n_eff, 
 {
     const uint32_t t = threadIdx.x;
     const bool active = t * ITEMS < n_eff;
+    // this thread's ITEMS consecutive elements: pidx(t*ITEMS + i) =
+    // t*(ITEMS+1) + i, one base address plus constant offsets
+    U *const mk = sk + t * (ITEMS + 1);
+    uint32_t *const mi = si + t * (ITEMS + 1);
     U k[ITEMS];
     uint32_t v[ITEMS];
     if (active) {
 #pragma unroll
         for (int i = 0; i < ITEMS; i++) {
-            k[i] = sk[pidx<ITEMS>(t * ITEMS + i)];
-            if (IDX) v[i] = si[pidx<ITEMS>(t * ITEMS + i)];
+            k[i] = mk[i];
+            if (IDX) v[i] = mi[i];
         }
 #pragma unroll
         for (int r = 0; r < ITEMS; r++) {
@@ -201,8 +205,8 @@ __device__ __forceinline__ void block_sort(U *sk, uint32_t *si, uint32_t n_eff, 
         if (active) {
 #pragma unroll
             for (int i = 0; i < ITEMS; i++) {
-                sk[pidx<ITEMS>(t * ITEMS + i)] = k[i];
-                if (IDX) si[pidx<ITEMS>(t * ITEMS + i)] = v[i];
+                mk[i] = k[i];
+                if (IDX) mi[i] = v[i];
             }
         }
         block_sync();
@@ -236,22 +240,21 @@ __device__ __forceinline__ void block_sort(U *sk, uint32_t *si, uint32_t n_eff, 
                 bend[h] = B + run;
             }
             // serial stable merges; ties take A.  One LDS load per step (the
-            // successor of the element taken); source positions are kept so
-            // the payloads are gathered afterwards.  Positions A+run and
-            // B+run stay inside the padded array.
+            // successor of the element taken) plus the payload of the
+            // element taken.  Positions A+run and B+run stay inside the
+            // padded array.
 #pragma unroll
             for (int h = 0; h < CH; h++) {
                 ka[h] = sk[pidx<ITEMS>(ai[h])];
                 kb[h] = sk[pidx<ITEMS>(bi[h])];
             }
-            uint32_t src[ITEMS];
 #pragma unroll
             for (int i = 0; i < PER; i++) {
 #pragma unroll
                 for (int h = 0; h < CH; h++) {
                     const bool takeA = (bi[h] >= bend[h]) || (ai[h] < aend[h] && !(kb[h] < ka[h]));
                     k[h * PER + i] = takeA ? ka[h] : kb[h];
-                    src[h * PER + i] = takeA ? ai[h] : bi[h];
+                    if (IDX) v[h * PER + i] = si[pidx<ITEMS>(takeA ? ai[h] : bi[h])];
                     ai[h] += takeA ? 1u : 0u;
                     bi[h] += takeA ? 0u : 1u;
                     const U nk = sk[pidx<ITEMS>(takeA ? ai[h] : bi[h])];
@@ -259,18 +262,14 @@ __device__ __forceinline__ void block_sort(U *sk, uint32_t *si, uint32_t n_eff, 
                     kb[h] = takeA ? kb[h] : nk;
                 }
             }
-            if (IDX) {
-#pragma unroll
-                for (int i = 0; i < ITEMS; i++) v[i] = si[pidx<ITEMS>(src[i])];
-            }
         }
     }
     block_sync();
     if (active) {
 #pragma unroll
         for (int i = 0; i < ITEMS; i++) {
-            sk[pidx<ITEMS>(t * ITEMS + i)] = k[i];
-            if (IDX) si[pidx<ITEMS>(t * ITEMS + i)] = v[i];
+            mk[i] = k[i];
+            if (IDX) mi[i] = v[i];
         }
     }
     block_sync();

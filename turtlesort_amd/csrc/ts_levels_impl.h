@@ -63,13 +63,14 @@ struct Cfg {
     static constexpr uint32_t SAMPLE = 1024;
     static constexpr uint32_t PP = CAP / SAMPLE;          // segments per pivot workgroup step
     static constexpr uint32_t IB = IDX ? 4 : 0;
-    static constexpr int CMIN = ilog2(SI), CMAX = ilog2(CAP);
+    // LDS-sort size classes c (n_eff = 2^c) from 64 keys up to CAP
+    static constexpr int CMIN = 6, CMAX = ilog2(CAP);
     static constexpr uint32_t SORT_KEY_BYTES = (padded_len<SI>(CAP) * KB + 15) & ~15u;
     static constexpr uint32_t SORT_BYTES = SORT_KEY_BYTES + (IDX ? padded_len<SI>(CAP) * 4 : 0);
     static constexpr uint32_t SCAT_BYTES = TILE * KB + TILE * IB + TILE;
     static_assert((S + 1) == (1 << LOG), "S + 1 must be a power of two");
     static_assert(S <= MAX_SPLIT && NBP <= WAVE, "too many splitters");
-    static_assert(CMAX < LDS_CLASSES, "size classes");
+    static_assert(CMAX < LDS_CLASSES && (1 << CMIN) >= SI, "size classes");
 };
 
 // ---- buffers by depth -----------------------------------------------------------
@@ -244,18 +245,25 @@ struct LevelK {
                 rec[threadIdx.x] = r;
             }
             block_sync();
-            for (uint32_t i = threadIdx.x; i < CAP; i += NT) {
-                const uint32_t q = i / SAMPLE, j = i % SAMPLE;
-                U v = key_max<U>();
-                if (q < nh) {
-                    const TaskRec r = rec[q];
-                    const uint32_t m = r.len < SAMPLE ? r.len : SAMPLE;
-                    if (j < m) {
-                        const uint64_t pos = ((2ull * j + 1ull) * (uint64_t)r.len) / (2ull * m);
-                        v = (U)(src_keys<U>(p, r.depth)[r.begin + (uint32_t)pos] ^ flip);
+            {
+                U kv[SI];
+#pragma unroll
+                for (int r = 0; r < SI; r++) {
+                    const uint32_t i = threadIdx.x + (uint32_t)r * NT;
+                    const uint32_t q = i / SAMPLE, j = i % SAMPLE;
+                    kv[r] = key_max<U>();
+                    if (q < nh) {
+                        const TaskRec tr = rec[q];
+                        const uint32_t m = tr.len < SAMPLE ? tr.len : SAMPLE;
+                        if (j < m) {
+                            const uint64_t pos = ((2ull * j + 1ull) * (uint64_t)tr.len) / (2ull * m);
+                            kv[r] = (U)(src_keys<U>(p, tr.depth)[tr.begin + (uint32_t)pos] ^ flip);
+                        }
                     }
                 }
-                sk[pidx<SI>(i)] = v;
+                U *const mk = sk + pidx<SI>(threadIdx.x);   // pidx(tid + r*NT) = pidx(tid) + r*(NT + NT/SI)
+#pragma unroll
+                for (int r = 0; r < SI; r++) mk[r * (NT + NT / SI)] = kv[r];
             }
             block_sync();
             block_sort<U, false, NT, SI>(sk, nullptr, CAP, SAMPLE);
@@ -628,17 +636,174 @@ struct LevelK {
         }
     }
 
-    static __device__ void lds(const Params &p, unsigned char *arena, TaskRec *rec, unsigned long long *acc)
+    // ---------------------------------------------------------------- LDS radix
+    // Tasks of class c >= CRAD fill whole waves: wave w's WS slots belong to
+    // task w >> (c - CRAD).  Their keys span a narrow range (they lie
+    // between two splitters of their parent), so a stable LSD radix sort of
+    // key - min over the range's bits takes a few passes of RW-bit digits,
+    // each pass ranking the keys with wave64 match masks and per-wave digit
+    // counters; padding slots take the largest digit in every pass and so
+    // stay behind the task's keys.
+    static constexpr int RW = 9;
+    static constexpr uint32_t NBIN = 1u << RW;
+    static constexpr uint32_t WS = (uint32_t)SI * WAVE;
+    static constexpr uint32_t CRAD = (uint32_t)ilog2(WS);
+    static constexpr uint32_t MAXPASS = (8 * KB + RW - 1) / RW;   // covers any key range
+    static constexpr uint32_t EPT = NW * NBIN / NT;   // scan entries per thread
+
+    struct RadixShared {
+        uint16_t cnt[NW][NBIN];   // per-wave digit counts, then the digits' first slots
+        U wmin[NW], wmax[NW];
+        U tmin[NW];
+        uint32_t red[16];
+        uint32_t maxbits;
+    };
+
+    static __device__ __forceinline__ U wave_min(U x)
+    {
+#pragma unroll
+        for (int d = WAVE / 2; d >= 1; d >>= 1) {
+            const U o = (U)__shfl_xor(x, d, WAVE);
+            x = o < x ? o : x;
+        }
+        return x;
+    }
+    static __device__ __forceinline__ U wave_max(U x)
+    {
+#pragma unroll
+        for (int d = WAVE / 2; d >= 1; d >>= 1) {
+            const U o = (U)__shfl_xor(x, d, WAVE);
+            x = o > x ? o : x;
+        }
+        return x;
+    }
+
+    static __device__ void radix(U *sk, uint32_t *si, const TaskRec *rec, uint32_t c, uint32_t tid, RadixShared &rs)
+    {
+        const uint32_t w = tid / WAVE, l = tid % WAVE;
+        const uint32_t wl = c - CRAD;                     // log2(waves per task)
+        const uint32_t q = w >> wl;
+        const uint32_t j0 = (w & ((1u << wl) - 1)) * WS;  // first slot of this wave inside its task
+        const uint32_t len = rec[q].len;
+        const uint32_t s0 = w * WS + l;                   // slot of item i: s0 + i*WAVE
+        // pidx(s0 + i*WAVE) = pidx(s0) + i*(WAVE + WAVE/SI): one base, constant offsets
+        constexpr uint32_t ISTR = WAVE + WAVE / SI;
+        U *const rk = sk + pidx<SI>(s0);
+        uint32_t *const ri = si + pidx<SI>(s0);
+        if (tid == 0) rs.maxbits = 0;
+        U mn = key_max<U>(), mx = 0;
+#pragma unroll
+        for (int i = 0; i < SI; i++) {
+            if (j0 + (uint32_t)i * WAVE + l < len) {
+                const U x = rk[i * ISTR];
+                mn = x < mn ? x : mn;
+                mx = x > mx ? x : mx;
+            }
+        }
+        mn = wave_min(mn);
+        mx = wave_max(mx);
+        if (l == 0) {
+            rs.wmin[w] = mn;
+            rs.wmax[w] = mx;
+        }
+        block_sync();
+        if (tid < (uint32_t)(NW >> wl)) {
+            U a = key_max<U>(), b = 0;
+            for (uint32_t k = 0; k < (1u << wl); k++) {
+                const U x = rs.wmin[(tid << wl) + k], y = rs.wmax[(tid << wl) + k];
+                a = x < a ? x : a;
+                b = y > b ? y : b;
+            }
+            rs.tmin[tid] = a;
+            const uint64_t span = b > a ? (uint64_t)(U)(b - a) : 0ull;
+            const uint32_t bits = span ? (uint32_t)(64 - __clzll(span)) : 0u;
+            atomicMax(&rs.maxbits, bits);
+        }
+        block_sync();
+        uint32_t npass = (rs.maxbits + RW - 1) / RW;
+        npass = npass < MAXPASS ? npass : MAXPASS;
+        const U tmin = rs.tmin[q];
+        for (uint32_t pass = 0; pass < npass; pass++) {
+            const uint32_t sh = pass * RW;
+            U k[SI];
+            uint32_t v[SI], dr[SI];
+#pragma unroll
+            for (int i = 0; i < SI; i++) {
+                k[i] = rk[i * ISTR];
+                if (IDX) v[i] = ri[i * ISTR];
+            }
+            for (uint32_t d = l; d < NBIN; d += WAVE) rs.cnt[w][d] = 0;
+            wave_sync();
+#pragma unroll
+            for (int i = 0; i < SI; i++) {
+                const bool real = j0 + (uint32_t)i * WAVE + l < len;
+                const uint32_t d = real ? (uint32_t)((U)(k[i] - tmin) >> sh) & (NBIN - 1) : NBIN - 1;
+                const uint64_t peers = match_peers<RW>(d, true);
+                const uint32_t r = mask_rank(peers);
+                const uint32_t old = rs.cnt[w][d];
+                if (r == 0) rs.cnt[w][d] = (uint16_t)(old + (uint32_t)__popcll(peers));
+                dr[i] = d | ((old + r) << 16);
+            }
+            block_sync();
+            // exclusive scan over (task, digit, wave of the task): the first
+            // slot of every (wave, digit) run in the step
+            {
+                uint32_t e0 = tid * EPT, cv[EPT], loc = 0;
+#pragma unroll
+                for (int k2 = 0; k2 < (int)EPT; k2++) {
+                    const uint32_t e = e0 + k2;
+                    const uint32_t ww = ((e >> (wl + RW)) << wl) + (e & ((1u << wl) - 1));
+                    const uint32_t d = (e >> wl) & (NBIN - 1);
+                    cv[k2] = rs.cnt[ww][d];
+                    loc += cv[k2];
+                }
+                uint32_t tot;
+                uint32_t run = block_excl_scan<NT>(loc, rs.red, &tot);
+#pragma unroll
+                for (int k2 = 0; k2 < (int)EPT; k2++) {
+                    const uint32_t e = e0 + k2;
+                    const uint32_t ww = ((e >> (wl + RW)) << wl) + (e & ((1u << wl) - 1));
+                    const uint32_t d = (e >> wl) & (NBIN - 1);
+                    rs.cnt[ww][d] = (uint16_t)run;
+                    run += cv[k2];
+                }
+            }
+            block_sync();
+#pragma unroll
+            for (int i = 0; i < SI; i++) {
+                const uint32_t pos = rs.cnt[w][dr[i] & 0xffffu] + (dr[i] >> 16);
+                if (pos < CAP) {
+                    sk[pidx<SI>(pos)] = k[i];
+                    if (IDX) si[pidx<SI>(pos)] = v[i];
+                }
+            }
+            block_sync();
+        }
+    }
+
+    // RAD: the workgroup steps of classes [CRAD, CMAX] (radix), else those
+    // of classes [CMIN, CRAD) (merge sort); two launches, disjoint tasks
+    template <bool RAD>
+    static __device__ void lds(const Params &p, unsigned char *arena, TaskRec *rec, RadixShared &rs,
+                               unsigned long long *acc)
     {
         U *sk = (U *)arena;
         uint32_t *si = (uint32_t *)(arena + C::SORT_KEY_BYTES);
         const U flip = (U)p.flip;
         U *dst = (U *)p.kout;
         const Ctrl *cc = p.ctrl;
-        const uint32_t total = cc->cls_wg[LDS_CLASSES];
+        const uint32_t cr = CRAD < (uint32_t)C::CMAX + 1 ? CRAD : (uint32_t)C::CMAX + 1;
+        const uint32_t vbeg = RAD ? cc->cls_wg[cr] : 0u;
+        const uint32_t vend = RAD ? cc->cls_wg[LDS_CLASSES] : cc->cls_wg[cr];
         uint64_t nbytes = 0, ntasks = 0;
         if (threadIdx.x == 0) acc[0] = acc[1] = 0;
-        for (uint32_t v = blockIdx.x; v < total; v += gridDim.x) {
+        for (uint32_t v = vbeg + blockIdx.x; v < vend; v += gridDim.x) {
+            // the thread index is re-read through an empty asm statement in
+            // every step: otherwise the compiler hoists the 2 x SI
+            // loop-invariant addresses derived from it out of the loop and
+            // spills them
+            uint32_t tid = threadIdx.x;
+            asm volatile("" : "+v"(tid));
             uint32_t c = C::CMIN;
             while (c < (uint32_t)C::CMAX && v >= cc->cls_wg[c + 1]) c++;
             const uint32_t n_eff = 1u << c, P = CAP >> c;
@@ -664,21 +829,51 @@ struct LevelK {
                 rec[q] = r;
             }
             block_sync();
-            for (uint32_t i = threadIdx.x; i < CAP; i += NT) {
-                const uint32_t q = i >> c, j = i & (n_eff - 1);
-                const TaskRec r = rec[q];
-                const bool on = j < r.len;
-                sk[pidx<SI>(i)] = on ? (U)(src_keys<U>(p, r.depth)[r.begin + j] ^ flip) : key_max<U>();
-                if (IDX) si[pidx<SI>(i)] = on ? src_idx(p, r.depth)[r.begin + j] : 0u;
+            // the global loads of a batch are issued before its LDS stores,
+            // so they are in flight together (batches of 8 bound the
+            // registers they hold)
+            for (int r0 = 0; r0 < SI; r0 += 8) {
+                U kv[8];
+                uint32_t iv[8];
+#pragma unroll
+                for (int r = 0; r < 8; r++) {
+                    const uint32_t i = tid + (uint32_t)(r0 + r) * NT;
+                    const uint32_t q = i >> c, j = i & (n_eff - 1);
+                    const TaskRec tr = rec[q];
+                    kv[r] = key_max<U>();
+                    iv[r] = 0u;
+                    if (j < tr.len) {
+                        kv[r] = (U)(src_keys<U>(p, tr.depth)[tr.begin + j] ^ flip);
+                        if (IDX) iv[r] = src_idx(p, tr.depth)[tr.begin + j];
+                    }
+                }
+                // pidx(tid + r*NT) = pidx(tid) + r*(NT + NT/SI)
+                U *const mk = sk + pidx<SI>(tid) + (uint32_t)r0 * (NT + NT / SI);
+                uint32_t *const mi = si + pidx<SI>(tid) + (uint32_t)r0 * (NT + NT / SI);
+#pragma unroll
+                for (int r = 0; r < 8; r++) {
+                    mk[r * (NT + NT / SI)] = kv[r];
+                    if (IDX) mi[r * (NT + NT / SI)] = iv[r];
+                }
             }
             block_sync();
-            block_sort<U, IDX, NT, SI>(sk, si, CAP, n_eff);
-            for (uint32_t i = threadIdx.x; i < CAP; i += NT) {
-                const uint32_t q = i >> c, j = i & (n_eff - 1);
-                const TaskRec r = rec[q];
-                if (j < r.len) {
-                    dst[r.begin + j] = (U)(sk[pidx<SI>(i)] ^ flip);
-                    if (IDX) p.iout[r.begin + j] = si[pidx<SI>(i)];
+            if constexpr (RAD) {
+                radix(sk, si, rec, c, tid, rs);
+            } else {
+                block_sort<U, IDX, NT, SI>(sk, si, CAP, n_eff);
+            }
+            {
+                const U *const mk = sk + pidx<SI>(tid);   // + r*(NT + NT/SI) for tid + r*NT
+                const uint32_t *const mi = si + pidx<SI>(tid);
+#pragma unroll
+                for (int r = 0; r < SI; r++) {
+                    const uint32_t i = tid + (uint32_t)r * NT;
+                    const uint32_t q = i >> c, j = i & (n_eff - 1);
+                    const TaskRec tr = rec[q];
+                    if (j < tr.len) {
+                        dst[tr.begin + j] = (U)(mk[r * (NT + NT / SI)] ^ flip);
+                        if (IDX) p.iout[tr.begin + j] = mi[r * (NT + NT / SI)];
+                    }
                 }
             }
             block_sync();
@@ -760,14 +955,15 @@ __global__ void __launch_bounds__(256) k_ldsbucket(Params p)
     LevelK<KB, IDX, S>::ldsbucket(p, hist, wbase);
 }
 
-template <int KB, bool IDX, int S>
+template <int KB, bool IDX, int S, bool RAD>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k_lds(Params p)
 {
     typedef LevelK<KB, IDX, S> K;
     __shared__ __align__(16) unsigned char arena[Cfg<KB, IDX, S>::SORT_BYTES];
-    __shared__ typename K::TaskRec rec[Cfg<KB, IDX, S>::CAP / Cfg<KB, IDX, S>::SI];
+    __shared__ typename K::TaskRec rec[Cfg<KB, IDX, S>::CAP >> Cfg<KB, IDX, S>::CMIN];
+    __shared__ typename K::RadixShared rs;
     __shared__ unsigned long long acc[2];
-    K::lds(p, arena, rec, acc);
+    K::template lds<RAD>(p, arena, rec, rs, acc);
 }
 
 // ---- host side ------------------------------------------------------------------------
@@ -789,7 +985,7 @@ hipError_t run_levels(const Params &p, int cus, int wg_per_cu, const uint32_t *d
     if (!o_tile) {
         o_cnt = occ<KB, IDX, S>((const void *)k_count<KB, IDX, S>, 512);
         o_pivot = occ<KB, IDX, S>((const void *)k_pivot<KB, IDX, S>, 512);
-        o_lds = occ<KB, IDX, S>((const void *)k_lds<KB, IDX, S>, 512);
+        o_lds = occ<KB, IDX, S>((const void *)k_lds<KB, IDX, S, true>, 512);
         o_tile = occ<KB, IDX, S>((const void *)k_scatter<KB, IDX, S>, 512);
     }
     auto lim = [&](int o) { return cus * ((wg_per_cu > 0 && wg_per_cu < o) ? wg_per_cu : o); };
@@ -819,7 +1015,8 @@ hipError_t run_levels(const Params &p, int cus, int wg_per_cu, const uint32_t *d
         TS_LAUNCH(K_LDSPREP, (k_ldscount<KB, IDX, S>), dim3(g_small), dim3(256), 0, st, p);
         TS_LAUNCH(K_LDSPREP, k_ldsplan, dim3(1), dim3(64), 0, st, p, (uint32_t)C::CMIN, (uint32_t)C::CMAX, C::CAP);
         TS_LAUNCH(K_LDSPREP, (k_ldsbucket<KB, IDX, S>), dim3(g_small), dim3(256), 0, st, p);
-        TS_LAUNCH(K_LDS, (k_lds<KB, IDX, S>), dim3(g_lds), dim3(512), 0, st, p);
+        TS_LAUNCH(K_LDS, (k_lds<KB, IDX, S, false>), dim3(g_lds), dim3(512), 0, st, p);
+        TS_LAUNCH(K_LDS, (k_lds<KB, IDX, S, true>), dim3(g_lds), dim3(512), 0, st, p);
     }
 #undef TS_LAUNCH
     return hipGetLastError();
