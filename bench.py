@@ -46,6 +46,10 @@ def parse():
     ap.add_argument("--splitters", type=int, default=0, help="override the engine's splitter count")
     ap.add_argument("--cutoff", type=int, default=0, help="override the LDS-sort cutoff")
     ap.add_argument("--tiles", type=int, default=0, help="override the tiles per workgroup step")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-segments comparison")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving run")
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
+    ap.add_argument("--serve-rate", type=float, default=2000.0)
     return ap.parse_args()
 
 
@@ -128,6 +132,92 @@ def check_sorted(torch, kin, kout, iin, iout):
     cnt = torch.bincount(iout.long(), minlength=kin.numel())
     ok &= bool((cnt == 1).all())
     return ok
+
+
+def bench_batched(torch, np, wl, TurtleSort, fill_keys, reps):
+    """20,000 uneven segments (log-uniform [16, 262144]) of int64 keys + int32
+    index, device-resident: one captured hipGraph replay of the segmented
+    engine run vs a loop of plain device sorts (turtle_sort_device, the
+    device half of StartSorting/Sync) over the same segments."""
+    lens = wl.loguniform_lengths(wl.SEED_C4_LENGTHS, wl.C4_SEGMENTS, wl.C4_LO, wl.C4_HI)
+    off = wl.segment_offsets(lens)
+    n = int(off[-1])
+    sorter = TurtleSort(np.int64, n)
+    kin = torch.empty(n, dtype=torch.int64, device="cuda")
+    fill_keys(kin, wl.SEED_C4_KEYS, wl.SHIFT_63BIT)
+    iin = torch.arange(n, dtype=torch.int32, device="cuda")
+    kout = torch.empty_like(kin)
+    iout = torch.empty_like(iin)
+    g = sorter.segmented_graph(kin, kout, off, iin, iout)
+    g.launch()
+    st = sorter.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g.launch()
+    torch.cuda.synchronize()
+    t_graph = (time.perf_counter() - t0) / reps
+    st = sorter.synchronize()
+    # validity: per-segment sorted and stable, a permutation inside each segment
+    seg_id = torch.repeat_interleave(torch.arange(len(lens), device="cuda"),
+                                     torch.from_numpy(lens).to("cuda"))
+    same = seg_id[1:] == seg_id[:-1]
+    ok = bool((~same | (kout[1:] >= kout[:-1])).all())
+    ok &= bool((~same | (kout[1:] != kout[:-1]) | (iout[1:] > iout[:-1])).all())
+    ok &= bool(torch.equal(kin[iout.long()], kout)) and bool(torch.equal(seg_id[iout.long()], seg_id))
+    del seg_id, same
+    g.close()
+    # the same segments through the segmented call without a graph
+    sorter.segmented_sort_device(kin, kout, off, iin, iout)
+    sorter.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sorter.segmented_sort_device(kin, kout, off, iin, iout)
+    sorter.synchronize()
+    t_seg = time.perf_counter() - t0
+    # loop of plain sorts, one call per segment
+    ko = torch.empty_like(kin)
+    io = torch.empty_like(iin)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(len(lens)):
+        a, b = int(off[s]), int(off[s + 1])
+        sorter.sort_device(kin[a:b], ko[a:b], iin[a:b], io[a:b])
+    sorter.synchronize()
+    torch.cuda.synchronize()
+    t_loop = time.perf_counter() - t0
+    ok &= bool(torch.equal(ko, kout)) and bool(torch.equal(io, iout))
+    sorter.close()
+    return {
+        "workload": "20,000 segments log-uniform [16, 262144], int64 keys uniform [0,2^63) + int32 index, "
+                    "device-resident",
+        "keys": n, "segments": len(lens),
+        "batched_graph_Gkeys_s": round(n / t_graph / 1e9, 4), "batched_graph_ms": round(t_graph * 1e3, 3),
+        "batched_call_Gkeys_s": round(n / t_seg / 1e9, 4),
+        "plain_loop_Gkeys_s": round(n / t_loop / 1e9, 4), "plain_loop_ms": round(t_loop * 1e3, 1),
+        "batched_vs_plain": round(t_loop / t_graph, 2),
+        "alg_bytes": int(st["bytes_count"] + st["bytes_scatter"] + st["bytes_lds"]),
+        "hbm_frac": round((st["bytes_count"] + st["bytes_scatter"] + st["bytes_lds"]) / t_graph / 1e9
+                          / HBM_PEAK_GBS, 4),
+        "valid": ok,
+    }
+
+
+def bench_serving(seconds, rate):
+    """open-loop request stream through the server (examples/serve_load)"""
+    import subprocess
+    exe = os.path.join(ROOT, "examples", "build", "serve_load")
+    if not os.path.exists(exe):
+        return {"error": f"{exe} not built"}
+    cmd = [exe, "--rate", str(rate), "--seconds", str(seconds), "--pool-gb", "64"]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=seconds * 4 + 120)
+    lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
+    if r.returncode != 0 or not lines:
+        return {"error": f"serve_load exit {r.returncode}", "stderr": r.stderr[-400:], "stdout": r.stdout[-400:]}
+    out = json.loads(lines[-1])
+    out["workload"] = (f"Poisson arrivals {rate:g} req/s for {seconds:g} s; sizes log-uniform [1K, 4M]; "
+                       "dtype uniform {uint8,int16,int32,int64}; int32 index on 50%; 64 GB device pool")
+    return out
 
 
 def main():
@@ -235,6 +325,16 @@ def main():
                               "hbm_frac": round(b2 / (a2 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                               "passes_equiv": round(b2 / floor_bytes, 3), "valid": ok2}
         out["c2_variants"] = variants
+
+    if not args.no_batched:
+        del kin, kout, iin, iout
+        sorter.close()
+        torch.cuda.empty_cache()
+        out["batched"] = bench_batched(torch, np, wl, TurtleSort, fill_keys, 3)
+        torch.cuda.empty_cache()
+
+    if not args.no_serve:
+        out["serving"] = bench_serving(args.serve_seconds, args.serve_rate)
 
     if not args.no_cpu:
         from oracle import oracle

@@ -181,6 +181,51 @@ int turtle_get_profile(turtle_handle h, turtle_profile *out);
 int turtle_get_tuning(turtle_handle h, turtle_tuning *t);
 int turtle_set_tuning(turtle_handle h, const turtle_tuning *t);
 
+/* ---- request loop with continuous batching ------------------------------ */
+/* No reference counterpart: a reference TurtleSort object holds one sort in
+ * flight (turtle-sort.cuh:53-61).  A server carves a device memory budget
+ * into batch slots (per served key type, slots_per_dtype sorters with their
+ * own HIP stream and buffers for slot_keys keys); a slot that is free takes
+ * every pending request of its type that fits, in arrival order, and sorts
+ * them as the segments of one engine run.  Batches of different slots run
+ * concurrently on the device.  Requests are host arrays sorted in place,
+ * like StartSorting; idx may be NULL.  All calls are thread-safe. */
+typedef struct turtle_server *turtle_server_handle;
+
+typedef struct turtle_server_config {
+    int64_t pool_bytes;        /* device memory budget of all slots (0: 64 GiB) */
+    int64_t slot_keys;         /* keys per batch slot = the largest request accepted
+                                  (0: from the budget, at most 32M) */
+    int32_t slots_per_dtype;   /* batches of one key type in flight at once (0: 2) */
+    int32_t dtype_mask;        /* bit (1 << dtype) for every served key type (0: all) */
+} turtle_server_config;
+
+typedef struct turtle_server_stats {
+    uint64_t submitted, completed, rejected, failed, batches, keys;
+    uint64_t pool_bytes, slot_keys;
+    double busy_seconds;           /* summed batch time of all slots */
+    uint32_t slots, in_flight_batches, max_in_flight_batches, pad;
+} turtle_server_stats;
+
+/* Allocates every slot (device buffers, pinned host buffers) and starts one
+ * worker thread per slot.  TURTLE_ERR_BUDGET if the slots do not fit the
+ * pool. */
+int turtle_server_create(const turtle_server_config *cfg, turtle_server_handle *out);
+/* Queues a request (returns at once with its ticket).  TURTLE_ERR_BUDGET if
+ * n exceeds slot_keys; TURTLE_ERR_DTYPE if the type is not served.  keys
+ * (and idx) must stay alive and untouched until the ticket is waited for. */
+int turtle_server_submit(turtle_server_handle s, int dtype, void *keys, int32_t *idx, int64_t n,
+                         uint64_t *ticket);
+/* Blocks until the request is sorted; *latency_seconds = completion time -
+ * submission time.  Returns the request's status.  A ticket is waited for
+ * once. */
+int turtle_server_wait(turtle_server_handle s, uint64_t ticket, double *latency_seconds);
+/* *done = 1 if the request has finished (then wait returns at once). */
+int turtle_server_test(turtle_server_handle s, uint64_t ticket, int *done);
+int turtle_server_get_stats(turtle_server_handle s, turtle_server_stats *out);
+/* Finishes every queued request, stops the workers, frees the slots. */
+int turtle_server_destroy(turtle_server_handle s);
+
 /* ---- synthetic inputs on the device (benchmarks and tests) -------------- */
 
 /* keys[i] = (splitmix64 draw i of seed) >> shift, truncated to the dtype

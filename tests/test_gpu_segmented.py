@@ -68,3 +68,38 @@ def test_segmented_config4_shape(torch_dev):
     assert bool((torch.bincount(iout.long(), minlength=n) == 1).all())
     assert st["error"] == 0
     sorter.close()
+
+
+def test_segmented_graph_replays(orc, torch_dev):
+    """The hipGraph form replayed several times, with other device work and
+    host synchronisation between replays; every replay equals the eager run
+    and the oracle."""
+    torch = torch_dev
+    lens = wl.loguniform_lengths(31, 300, 16, 100_000)
+    off = wl.segment_offsets(lens)
+    n = int(off[-1])
+    keys = wl.gen_keys(np.int64, n, 37, wl.SHIFT_63BIT)
+    sorter = TurtleSort(np.int64, n)
+    kin = h2d(torch, keys)
+    iin = torch.arange(n, dtype=torch.int32, device="cuda")
+    kout = torch.empty_like(kin)
+    iout = torch.empty_like(iin)
+    g = sorter.segmented_graph(kin, kout, off, iin, iout)   # captured on a fresh handle
+    k_eager = torch.empty_like(kin)
+    i_eager = torch.empty_like(iin)
+    sorter.segmented_sort_device(kin, k_eager, off, iin, i_eager)
+    sorter.synchronize()
+    check_segments(orc, keys, off, d2h(torch, k_eager), d2h(torch, i_eager))
+    for rep in range(4):
+        print(f"case graph replay {rep}", flush=True)
+        kout.fill_(-1)
+        iout.fill_(-1)
+        torch.cuda.synchronize()
+        g.launch()
+        st = sorter.synchronize()
+        assert st["error"] == 0
+        assert torch.equal(kout, k_eager) and torch.equal(iout, i_eager), f"replay {rep} differs"
+        _ = (kin * 3).sum()                          # eager work between replays
+        torch.cuda.synchronize()
+    g.close()
+    sorter.close()

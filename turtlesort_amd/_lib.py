@@ -79,6 +79,29 @@ class Profile(ctypes.Structure):
         return {k: {"ms": self.ms[i], "launches": self.launches[i]} for i, k in enumerate(KERNEL_KINDS)}
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("pool_bytes", ctypes.c_int64),
+        ("slot_keys", ctypes.c_int64),
+        ("slots_per_dtype", ctypes.c_int32),
+        ("dtype_mask", ctypes.c_int32),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [
+        ("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("rejected", ctypes.c_uint64),
+        ("failed", ctypes.c_uint64), ("batches", ctypes.c_uint64), ("keys", ctypes.c_uint64),
+        ("pool_bytes", ctypes.c_uint64), ("slot_keys", ctypes.c_uint64),
+        ("busy_seconds", ctypes.c_double),
+        ("slots", ctypes.c_uint32), ("in_flight_batches", ctypes.c_uint32),
+        ("max_in_flight_batches", ctypes.c_uint32), ("pad", ctypes.c_uint32),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int
 _i64 = ctypes.c_int64
@@ -103,6 +126,12 @@ SIGNATURES = {
     "turtle_get_profile": (_i32, [_vp, ctypes.POINTER(Profile)]),
     "turtle_get_tuning": (_i32, [_vp, ctypes.POINTER(Tuning)]),
     "turtle_set_tuning": (_i32, [_vp, ctypes.POINTER(Tuning)]),
+    "turtle_server_create": (_i32, [ctypes.POINTER(ServerConfig), ctypes.POINTER(_vp)]),
+    "turtle_server_submit": (_i32, [_vp, _i32, _vp, _vp, _i64, ctypes.POINTER(_u64)]),
+    "turtle_server_wait": (_i32, [_vp, _u64, ctypes.POINTER(ctypes.c_double)]),
+    "turtle_server_test": (_i32, [_vp, _u64, ctypes.POINTER(_i32)]),
+    "turtle_server_get_stats": (_i32, [_vp, ctypes.POINTER(ServerStats)]),
+    "turtle_server_destroy": (_i32, [_vp]),
     "turtle_fill_keys": (_i32, [_i32, _vp, _i64, _u64, _i32, _vp]),
     "turtle_fill_iota": (_i32, [_vp, _i64, _vp]),
     "turtle_status_string": (ctypes.c_char_p, [_i32]),

@@ -142,6 +142,16 @@ class TurtleSort:
             self._h, _ptr(keys_in), _ptr(idx_in), _ptr(keys_out), _ptr(idx_out),
             off.ctypes.data_as(ctypes.c_void_p), off.size - 1, _stream(stream)))
 
+    def segmented_graph(self, keys_in, keys_out, offsets, idx_in=None, idx_out=None):
+        """hipGraph of one segmented engine run over these buffers
+        (turtle_segmented_graph_create); replay with .launch()."""
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        g = ctypes.c_void_p()
+        _lib.check(self._lib.turtle_segmented_graph_create(
+            self._h, _ptr(keys_in), _ptr(idx_in), _ptr(keys_out), _ptr(idx_out),
+            off.ctypes.data_as(ctypes.c_void_p), off.size - 1, ctypes.byref(g)))
+        return SegmentGraph(self._lib, g)
+
     def multi_sort_device(self, arr_size, keys_in, keys_out, stream=None):
         num = keys_in.numel() // arr_size
         _lib.check(self._lib.turtle_multisort_device(
@@ -176,6 +186,28 @@ class TurtleSort:
         _lib.check(self._lib.turtle_set_tuning(self._h, ctypes.byref(t)))
 
 
+class SegmentGraph:
+    """A captured segmented sort; launch() replays it on a stream."""
+
+    def __init__(self, lib, g):
+        self._lib = lib
+        self._g = g
+
+    def launch(self, stream=None):
+        _lib.check(self._lib.turtle_graph_launch(self._g, _stream(stream)))
+
+    def close(self):
+        if getattr(self, "_g", None) is not None and self._g.value:
+            self._lib.turtle_graph_destroy(self._g)
+            self._g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def fill_keys(keys, seed, shift, stream=None):
     """keys[i] = splitmix64 draw i of seed >> shift (device tensor, in place)."""
     lib = _lib.load()
@@ -186,3 +218,64 @@ def fill_keys(keys, seed, shift, stream=None):
 def fill_iota(idx, stream=None):
     lib = _lib.load()
     _lib.check(lib.turtle_fill_iota(_ptr(idx), idx.numel(), _stream(stream)))
+
+
+class SortServer:
+    """Request loop with continuous batching (turtle_server_* of the C ABI):
+    submit() queues a host array (sorted in place, like start_sorting) and
+    returns a ticket; wait(ticket) blocks until it is sorted and returns the
+    latency in seconds.  The arrays must stay alive until waited for."""
+
+    def __init__(self, pool_bytes=0, slot_keys=0, slots_per_dtype=0, dtypes=None):
+        self._lib = _lib.load()
+        cfg = _lib.ServerConfig()
+        cfg.pool_bytes = int(pool_bytes)
+        cfg.slot_keys = int(slot_keys)
+        cfg.slots_per_dtype = int(slots_per_dtype)
+        cfg.dtype_mask = 0 if dtypes is None else sum(1 << dtype_code(d) for d in dtypes)
+        h = ctypes.c_void_p()
+        _lib.check(self._lib.turtle_server_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+        self._arrays = {}
+
+    def submit(self, keys, idx=None):
+        if not isinstance(keys, np.ndarray) or not keys.flags.c_contiguous:
+            raise TypeError("keys must be a C-contiguous numpy array")
+        if idx is not None and (idx.dtype != np.int32 or not idx.flags.c_contiguous or idx.size != keys.size):
+            raise TypeError("idx must be a C-contiguous int32 array of the keys' length")
+        t = ctypes.c_uint64(0)
+        _lib.check(self._lib.turtle_server_submit(
+            self._h, dtype_code(keys.dtype), keys.ctypes.data_as(ctypes.c_void_p),
+            None if idx is None else idx.ctypes.data_as(ctypes.c_void_p), keys.size, ctypes.byref(t)))
+        self._arrays[t.value] = (keys, idx)
+        return t.value
+
+    def wait(self, ticket):
+        lat = ctypes.c_double(0.0)
+        try:
+            _lib.check(self._lib.turtle_server_wait(self._h, ctypes.c_uint64(ticket), ctypes.byref(lat)))
+        finally:
+            self._arrays.pop(ticket, None)
+        return lat.value
+
+    def stats(self):
+        st = _lib.ServerStats()
+        _lib.check(self._lib.turtle_server_get_stats(self._h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.turtle_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -896,7 +896,9 @@ int turtle_segmented_graph_create(turtle_handle h, const void *keys_in, const in
     std::vector<uint32_t> off((size_t)num_segments + 1);
     for (size_t s = 0; s < off.size(); s++) off[s] = (uint32_t)offsets[s];
     h->host_off = off;   // the level estimate reads the longest segment from here
+    // everything the run allocates is allocated before the capture starts
     int rc = ensure_tables(h, (uint64_t)total, (uint32_t)num_segments, 1);
+    if (!rc) rc = ensure_readback(h);
     hipError_t e = hipSuccess;
     if (!rc) e = hipMalloc((void **)&g->dev_off, off.size() * sizeof(uint32_t));
     if (!rc && e == hipSuccess) {

@@ -981,13 +981,19 @@ hipError_t run_levels(const Params &p, int cus, int wg_per_cu, const uint32_t *d
                       LaunchTimer *tm)
 {
     typedef Cfg<KB, IDX, S> C;
-    static int o_tile = 0, o_cnt = 0, o_pivot = 0, o_lds = 0;
-    if (!o_tile) {
-        o_cnt = occ<KB, IDX, S>((const void *)k_count<KB, IDX, S>, 512);
-        o_pivot = occ<KB, IDX, S>((const void *)k_pivot<KB, IDX, S>, 512);
-        o_lds = occ<KB, IDX, S>((const void *)k_lds<KB, IDX, S, true>, 512);
-        o_tile = occ<KB, IDX, S>((const void *)k_scatter<KB, IDX, S>, 512);
-    }
+    // workgroups per CU of each kernel (computed once, thread-safe)
+    struct Occ {
+        int tile, cnt, pivot, lds;
+    };
+    static const Occ oc = [] {
+        Occ o;
+        o.cnt = occ<KB, IDX, S>((const void *)k_count<KB, IDX, S>, 512);
+        o.pivot = occ<KB, IDX, S>((const void *)k_pivot<KB, IDX, S>, 512);
+        o.lds = occ<KB, IDX, S>((const void *)k_lds<KB, IDX, S, true>, 512);
+        o.tile = occ<KB, IDX, S>((const void *)k_scatter<KB, IDX, S>, 512);
+        return o;
+    }();
+    const int o_tile = oc.tile, o_cnt = oc.cnt, o_pivot = oc.pivot, o_lds = oc.lds;
     auto lim = [&](int o) { return cus * ((wg_per_cu > 0 && wg_per_cu < o) ? wg_per_cu : o); };
     const int g_tile = lim(o_tile), g_cnt = lim(o_cnt), g_pivot = cus * o_pivot, g_lds = cus * o_lds;
     const int g_small = cus * 2;
