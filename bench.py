@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving run")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy 5 times (calibrates the HBM byte counters)")
     return ap.parse_args()
 
 
@@ -229,6 +231,15 @@ def main():
     import numpy as np
     import torch
     from turtlesort_amd import TurtleSort, fill_iota, fill_keys, workloads as wl
+
+    if args.calib_copy:
+        src = torch.ones(1 << 28, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calib_copy_bytes_read": src.numel() * 4, "calib_copy_bytes_written": dst.numel() * 4}))
+        return
 
     torch.cuda.set_device(0)
     n = wl.C2_N

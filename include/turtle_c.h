@@ -177,9 +177,22 @@ typedef struct turtle_profile {
 int turtle_set_profiling(turtle_handle h, int on);
 int turtle_get_profile(turtle_handle h, turtle_profile *out);
 
-/* Tuning of a handle (defaults are the autotuned table, or built-ins). */
+/* Tuning.  Unless turtle_set_tuning fixed it, every run picks its tuning
+ * from the autotuned table (tuning_table.txt next to the library, or
+ * $TURTLE_TUNING_FILE) by key width, index flag and size, else the built-in
+ * default.  turtle_get_tuning returns the tuning of the last run (before
+ * any run: the default). */
 int turtle_get_tuning(turtle_handle h, turtle_tuning *t);
 int turtle_set_tuning(turtle_handle h, const turtle_tuning *t);
+
+/* Autotuner (no reference counterpart: the reference fixes its thresholds
+ * at compile time, turtle-sort.cu:7,12 and turtle-sort.cuh:130): times the
+ * engine on n synthetic keys (uniform over the key type, iota index when
+ * with_index) in device memory for every candidate (splitters 7/15/31 x LDS
+ * cutoff cap/4, cap/2, cap x tiles per step 1/2), reps timed runs each, and
+ * returns the fastest in *best (and its ms per sort).  The handle's own
+ * tuning is left as it was. */
+int turtle_autotune(turtle_handle h, int64_t n, int with_index, int reps, turtle_tuning *best, double *best_ms);
 
 /* ---- request loop with continuous batching ------------------------------ */
 /* No reference counterpart: a reference TurtleSort object holds one sort in

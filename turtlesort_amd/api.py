@@ -178,6 +178,14 @@ class TurtleSort:
         _lib.check(self._lib.turtle_get_tuning(self._h, ctypes.byref(t)))
         return t.as_dict()
 
+    def autotune(self, n, with_index=True, reps=3):
+        """fastest tuning on n synthetic keys in device memory -> (dict, ms)"""
+        t = _lib.Tuning()
+        ms = ctypes.c_double(0.0)
+        _lib.check(self._lib.turtle_autotune(self._h, int(n), int(bool(with_index)), int(reps),
+                                             ctypes.byref(t), ctypes.byref(ms)))
+        return t.as_dict(), ms.value
+
     def set_tuning(self, **kw):
         t = _lib.Tuning()
         _lib.check(self._lib.turtle_get_tuning(self._h, ctypes.byref(t)))

@@ -8,8 +8,11 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string.h>
+#include <dlfcn.h>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
+#include <mutex>
 #include <cstdlib>
 #include <string>
 #include <vector>
@@ -160,7 +163,9 @@ struct turtle_sorter {
     uint32_t *dev_off = nullptr;
     size_t dev_off_cap = 0;
     std::vector<uint32_t> host_off;
-    turtle_tuning tune{};
+    turtle_tuning tune{};        // tuning of the current / last run
+    turtle_tuning tune_base{};   // built-in default, or what turtle_set_tuning set
+    bool tune_fixed = false;     // set by turtle_set_tuning: the tuning table is not consulted
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
     uint64_t last_n = 0;
@@ -303,6 +308,8 @@ hipStream_t stream_of(turtle_sorter *h, void *stream)
     return (hipStream_t)stream;
 }
 
+turtle_tuning tuning_for(const turtle_sorter *h, bool idx, uint64_t total);
+
 // One engine run over device buffers.  In-place requests (kin == kout) first
 // copy the input into the second ping-pong buffer, which depth 0 only reads.
 // The levels are enqueued in batches; after a batch one small pinned
@@ -314,6 +321,8 @@ int run_engine(turtle_sorter *h, const void *kin, const uint32_t *iin, void *kou
                const uint32_t *dev_off, uint32_t nseg, hipStream_t st, bool capturing = false)
 {
     const bool idx = iin != nullptr && iout != nullptr;
+    // (a capture uses the tuning its creator resolved and sized the tables for)
+    if (!capturing) h->tune = tuning_for(h, idx, total);
     int rc = ensure_tables(h, total, nseg, 1);
     if (rc) return rc;
     if ((rc = ensure_readback(h))) return rc;
@@ -447,6 +456,66 @@ turtle_tuning default_tuning(int kb)
     return t;
 }
 
+// ---- tuning table -------------------------------------------------------------
+// Written by the autotuner (turtlesort_amd/autotune.py over turtle_autotune):
+// lines "kb idx log2n splitters lds_cutoff tiles_per_task workgroups_per_cu
+// ms", '#' comments.  Read once from $TURTLE_TUNING_FILE, else from
+// tuning_table.txt next to this library.  A run of `total` keys uses the
+// entry of its key width and index flag whose log2n is nearest to
+// log2(total); without an entry the built-in default applies.
+struct TuneEntry {
+    int kb, idx, lg;
+    turtle_tuning t;
+};
+
+const std::vector<TuneEntry> &tuning_table()
+{
+    static std::vector<TuneEntry> tab;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        std::string path;
+        const char *env = getenv("TURTLE_TUNING_FILE");
+        if (env && *env) {
+            path = env;
+        } else {
+            Dl_info info;
+            if (dladdr((void *)&tuning_table, &info) && info.dli_fname) {
+                path = info.dli_fname;
+                const size_t slash = path.rfind('/');
+                path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/tuning_table.txt";
+            }
+        }
+        FILE *f = path.empty() ? nullptr : fopen(path.c_str(), "r");
+        if (!f) return;
+        char line[256];
+        while (fgets(line, sizeof line, f)) {
+            if (line[0] == '#') continue;
+            TuneEntry e{};
+            double ms = 0;
+            if (sscanf(line, "%d %d %d %d %d %d %d %lf", &e.kb, &e.idx, &e.lg, &e.t.splitters, &e.t.lds_cutoff,
+                       &e.t.tiles_per_task, &e.t.workgroups_per_cu, &ms) >= 7)
+                tab.push_back(e);
+        }
+        fclose(f);
+    });
+    return tab;
+}
+
+bool valid_tuning(const turtle_tuning &t, int kb);
+
+// tuning of a run over `total` keys (the handle's own when it was set by hand)
+turtle_tuning tuning_for(const turtle_sorter *h, bool idx, uint64_t total)
+{
+    if (h->tune_fixed) return h->tune;
+    const int lg = total > 1 ? (int)std::lround(std::log2((double)total)) : 0;
+    const TuneEntry *best = nullptr;
+    for (const TuneEntry &e : tuning_table()) {
+        if (e.kb != h->kb || e.idx != (idx ? 1 : 0) || !valid_tuning(e.t, h->kb)) continue;
+        if (!best || std::abs(e.lg - lg) < std::abs(best->lg - lg)) best = &e;
+    }
+    return best ? best->t : h->tune_base;
+}
+
 constexpr size_t STAGE_BYTES = 16u << 20;
 
 int ensure_staging(turtle_sorter *h)
@@ -559,7 +628,7 @@ int turtle_create(int dtype, int64_t max_n, int track_index, int compress, turtl
     h->flip = flip_of(dtype);
     h->track = track_index ? 1 : 0;
     h->max_n = max_n;
-    h->tune = default_tuning(kb);
+    h->tune = h->tune_base = default_tuning(kb);
     int rc = TURTLE_OK;
     hipDeviceProp_t pr;
     hipError_t e = hipGetDevice(&h->device);
@@ -680,7 +749,74 @@ int turtle_set_tuning(turtle_handle h, const turtle_tuning *t)
 {
     if (!h || !t) return fail(TURTLE_ERR_INVALID, "NULL argument");
     if (!valid_tuning(*t, h->kb)) return fail(TURTLE_ERR_INVALID, "tuning out of range");
-    h->tune = *t;
+    h->tune = h->tune_base = *t;
+    h->tune_fixed = true;
+    return TURTLE_OK;
+}
+
+int turtle_autotune(turtle_handle h, int64_t n, int with_index, int reps, turtle_tuning *best, double *best_ms)
+{
+    if (!h || !best) return fail(TURTLE_ERR_INVALID, "NULL argument");
+    if (n < 1 || n > h->max_n) return fail(TURTLE_ERR_CAPACITY, "n must be in [1, max_n]");
+    if (with_index && !h->track) return fail(TURTLE_ERR_INVALID, "handle created without index tracking");
+    if (reps < 1) reps = 3;
+    const bool idx = with_index != 0;
+    const size_t kbytes = (size_t)n * h->kb, ibytes = (size_t)n * 4;
+    void *kin = nullptr, *kout = nullptr;
+    uint32_t *iin = nullptr, *iout = nullptr;
+    hipStream_t st = h->stream;
+    int rc = TURTLE_OK;
+    auto cleanup = [&] {
+        void *b[] = {kin, kout, iin, iout};
+        for (void *x : b)
+            if (x) (void)hipFree(x);
+    };
+    if ((rc = alloc_buf(&kin, kbytes, "hipMalloc(autotune keys)")) || (rc = alloc_buf(&kout, kbytes, "hipMalloc")) ||
+        (idx && ((rc = alloc_buf((void **)&iin, ibytes, "hipMalloc")) || (rc = alloc_buf((void **)&iout, ibytes, "hipMalloc"))))) {
+        cleanup();
+        return rc;
+    }
+    // keys uniform over the key type's range, iota index
+    hipError_t e = ts::fill_keys(h->kb, kin, (uint64_t)n, 0x7e5eed01ull, 0, st);
+    if (e == hipSuccess && idx) e = ts::fill_iota(iin, (uint64_t)n, st);
+    if (e != hipSuccess) {
+        cleanup();
+        return fail_hip(e, "autotune fill");
+    }
+    const turtle_tuning base = h->tune_base;
+    const bool fixed = h->tune_fixed;
+    h->tune_fixed = true;
+    std::vector<turtle_tuning> cands;
+    const int cap = (int)ts::engine_cap(h->kb);
+    for (int S : {7, 15, 31})
+        for (int cut : {cap / 4, cap / 2, cap})
+            for (int tiles : {1, 2})
+                cands.push_back(turtle_tuning{S, cut, tiles, 0});
+    double best_t = 1e30;
+    turtle_tuning best_c = base;
+    for (const turtle_tuning &c : cands) {
+        h->tune = h->tune_base = c;
+        // one untimed run (tables sized for this tuning), then reps timed runs
+        rc = run_engine(h, kin, iin, kout, iout, (uint64_t)n, nullptr, 1, st);
+        if (!rc) rc = turtle_synchronize(h, nullptr);
+        if (rc) break;
+        const double t0 = now_s();
+        for (int r = 0; r < reps && !rc; r++) rc = run_engine(h, kin, iin, kout, iout, (uint64_t)n, nullptr, 1, st);
+        if (!rc) rc = turtle_synchronize(h, nullptr);
+        if (rc) break;
+        const double t = (now_s() - t0) / reps;
+        if (t < best_t) {
+            best_t = t;
+            best_c = c;
+        }
+    }
+    h->tune_base = base;
+    h->tune_fixed = fixed;
+    h->tune = base;
+    cleanup();
+    if (rc) return rc;
+    *best = best_c;
+    if (best_ms) *best_ms = best_t * 1e3;
     return TURTLE_OK;
 }
 
@@ -897,6 +1033,7 @@ int turtle_segmented_graph_create(turtle_handle h, const void *keys_in, const in
     for (size_t s = 0; s < off.size(); s++) off[s] = (uint32_t)offsets[s];
     h->host_off = off;   // the level estimate reads the longest segment from here
     // everything the run allocates is allocated before the capture starts
+    h->tune = tuning_for(h, idx_in != nullptr, (uint64_t)total);
     int rc = ensure_tables(h, (uint64_t)total, (uint32_t)num_segments, 1);
     if (!rc) rc = ensure_readback(h);
     hipError_t e = hipSuccess;

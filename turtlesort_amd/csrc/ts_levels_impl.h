@@ -858,11 +858,19 @@ struct LevelK {
             }
             block_sync();
             if constexpr (RAD) {
-                radix(sk, si, rec, c, tid, rs);
+                uint32_t tid3 = threadIdx.x;
+                asm volatile("" : "+v"(tid3));
+                radix(sk, si, rec, c, tid3, rs);
             } else {
                 block_sort<U, IDX, NT, SI>(sk, si, CAP, n_eff);
             }
             {
+                // a fresh opaque copy of the thread index: the slot numbers
+                // below are recomputed here instead of being kept live (in
+                // spilled registers) from the fill through the sort
+                uint32_t tid2 = threadIdx.x;
+                asm volatile("" : "+v"(tid2));
+                const uint32_t tid = tid2;
                 const U *const mk = sk + pidx<SI>(tid);   // + r*(NT + NT/SI) for tid + r*NT
                 const uint32_t *const mi = si + pidx<SI>(tid);
 #pragma unroll
