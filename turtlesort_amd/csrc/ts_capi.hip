@@ -279,8 +279,8 @@ ts::Params make_params(turtle_sorter *h, const void *kin, const uint32_t *iin, v
 
 // partition levels expected for a segment of n keys: the longest segment
 // shrinks about (S+1) times per level (a quarter more is allowed for uneven
-// splits) until it fits the cutoff, plus one spare level; the host reads
-// back after these and adds levels only if segments are left
+// splits) until it fits the cutoff; the host reads back after these and
+// adds levels only if segments are left
 uint32_t levels_estimate(uint64_t n, uint64_t cutoff, uint64_t S)
 {
     uint32_t L = 0;
@@ -288,7 +288,7 @@ uint32_t levels_estimate(uint64_t n, uint64_t cutoff, uint64_t S)
         n = (n * 5 / 4 + S) / (S + 1);
         L++;
     }
-    return L + 1;
+    return L;
 }
 
 int ensure_readback(turtle_sorter *h)
@@ -349,11 +349,17 @@ int run_engine(turtle_sorter *h, const void *kin, const uint32_t *iin, void *kou
     uint32_t nlev = levels_estimate(longest, (uint64_t)h->tune.lds_cutoff, (uint64_t)S);
     if (!capturing) HIPCHK(hipEventRecord(h->ev0, st), "hipEventRecord");
     if (capturing) {
-        nlev += 6;
+        nlev += 2;   // no read-back inside a graph: two spare levels
         if (nlev > (uint32_t)ts::MAX_LEVELS) nlev = ts::MAX_LEVELS;
         HIPCHK(ts::engine_levels(p, h->kb, idx, S, h->cus, wpc, dev_off, nseg, (uint32_t)total, 0, nlev, true, true, st),
                "engine launch");
         h->levels_run = nlev;
+    } else if (nlev == 0) {
+        // every root fits the LDS sort: no partition level, no read-back
+        HIPCHK(ts::engine_levels(p, h->kb, idx, S, h->cus, wpc, dev_off, nseg, (uint32_t)total, 0, 0, true, true, st,
+                                 h->timer),
+               "engine launch");
+        h->levels_run = 0;
     } else {
         uint32_t L0 = 0;
         bool first = true;
