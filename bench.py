@@ -55,7 +55,7 @@ def parse():
     return ap.parse_args()
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_scatter_64M_int32_idx_traffic.md")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_64M_int32_idx.md")
 # bytes of the algorithmic model per kernel kind (device counters)
 ALG_BYTES_OF = {"count": "bytes_count", "scatter": "bytes_scatter", "lds": "bytes_lds"}
 
@@ -96,13 +96,14 @@ def time_sorts(torch, sorter, kin, kout, iin, iout, steps, warmup, profile=True)
     return wall, dev_ms, st, prof
 
 
-def read_traffic():
-    """HBM bytes per launch of the scatter kernel from the committed
-    rocprofv3 counter summary (line 'traffic_bytes_per_launch: N')"""
+def read_traffic(kind):
+    """HBM bytes per launch of a kernel kind from the committed rocprofv3
+    counter summary (line 'traffic_bytes_per_launch[kind]: N')"""
+    key = f"traffic_bytes_per_launch[{kind}]:"
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
-                if line.strip().startswith("traffic_bytes_per_launch:"):
+                if line.strip().startswith(key):
                     return int(float(line.split(":", 1)[1].strip()))
     except OSError:
         pass
@@ -272,7 +273,7 @@ def main():
     drow = kernels[dom]
     d_bytes_launch = drow["alg_bytes_per_step"] / drow["launches_per_step"]
     d_achieved = d_bytes_launch / (drow["avg_launch_ms"] * 1e-3) / 1e9
-    traffic = read_traffic() if dom == "scatter" else None
+    traffic = read_traffic(dom)
     e2e = alg_bytes / (avg_dev_ms * 1e-3) / 1e9
 
     out = {
@@ -336,6 +337,21 @@ def main():
                               "hbm_frac": round(b2 / (a2 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                               "passes_equiv": round(b2 / floor_bytes, 3), "valid": ok2}
         out["c2_variants"] = variants
+
+    # the host-array API (StartSorting/Sync with pinned staging): PCIe included,
+    # reported beside the device-resident value, never as it
+    hk = wl.gen_keys(np.int32, n, wl.SEED_C2_UNIFORM, wl.SHIFT_31BIT)
+    hi = np.arange(n, dtype=np.int32)
+    api_s = []
+    for _ in range(3):
+        k = hk.copy()
+        x = hi.copy()
+        sorter.start_sorting(k, x)
+        api_s.append(sorter.sync())
+    out["api_host_arrays"] = {"workload": "StartSorting(&keys,&idx)+Sync() on host arrays, same 64M input",
+                              "Gkeys_s": round(n / min(api_s) / 1e9, 4), "seconds": [round(t, 4) for t in api_s],
+                              "valid": bool((k[1:] >= k[:-1]).all() and (hk[x] == k).all())}
+    del hk, hi, k, x
 
     if not args.no_batched:
         del kin, kout, iin, iout
