@@ -411,17 +411,138 @@ struct LevelK {
     }
 
     // ---------------------------------------------------------------- scatter
-    static __device__ void scatter(const Params &p, unsigned char *arena, SharedTile &sh)
+    // One tile of the scatter.  FULL: the tile holds TILE keys, so no
+    // per-element bounds predicates.  Consistency checks are branch-free
+    // (a flag per thread, reported once per tile).
+    template <bool FULL>
+    static __device__ __forceinline__ void scatter_tile(const Params &p, unsigned char *arena, SharedTile &sh,
+                                                        const Seg *sg, uint32_t t, const Reg &reg)
     {
-        const uint32_t ntot = p.ctrl->tile_alloc < p.tilecap ? p.ctrl->tile_alloc : p.tilecap;
-        const uint32_t G = p.tiles_per_wg;
         const U flip = (U)p.flip;
         U *stk = (U *)arena;
         uint32_t *sti = (uint32_t *)(arena + TILE * KB);
         unsigned char *stb = arena + TILE * KB + TILE * C::IB;
         U *kfin = (U *)p.kout;
         uint32_t *ifin = p.iout;
-        const int w = threadIdx.x / WAVE;
+        // the thread index is re-read through an empty asm statement in every
+        // tile: the element offsets and LDS addresses derived from it are then
+        // recomputed per tile instead of being hoisted out of the tile loop
+        // and spilled
+        uint32_t tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        const int w = __builtin_amdgcn_readfirstlane(tid / WAVE);
+        const uint32_t ln = tid % WAVE;
+        const uint32_t begin = sg->begin, len = sg->len, depth = sg->depth;
+        const uint32_t tb = sg->tile_base, ntl = sg->ntiles;
+        const U *src = src_keys<U>(p, depth) + begin;
+        const uint32_t *isrc = src_idx(p, depth) + begin;
+        U *knext = next_keys<U>(p, depth);
+        uint32_t *inext = next_idx(p, depth);
+        const uint32_t ts = t * TILE;
+        const uint32_t tl = FULL ? TILE : ((len - ts) < TILE ? (len - ts) : TILE);
+        bool bad = false;
+        if (ln < (uint32_t)NBP) sh.wcnt[w][ln] = 0;
+        wave_sync();
+        U x[TI];
+        uint32_t ix[TI];
+        uint32_t pk[TI];   // bin | rank inside the wave's bin << 8
+#pragma unroll
+        for (int c = 0; c < TI; c++) {
+            const uint32_t e = (uint32_t)w * (TI * WAVE) + c * WAVE + ln;
+            if (FULL) x[c] = src[ts + e];
+            else x[c] = e < tl ? src[ts + e] : (U)0;
+        }
+        // rank: running per-wave bin counter (one wave owns wcnt[w], its LDS
+        // accesses complete in program order) + rank among the lanes of this
+        // step with the same bin
+        // splitter searches in groups of 8 independent chains (a scheduling
+        // barrier between groups bounds the registers they hold)
+#pragma unroll
+        for (int c0 = 0; c0 < TI; c0 += 8) {
+#pragma unroll
+            for (int c = c0; c < c0 + 8 && c < TI; c++) pk[c] = classify<U, S, LOG>((U)(x[c] ^ flip), sh.sp, reg);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+#pragma unroll
+        for (int c = 0; c < TI; c++) {
+            const uint32_t e = (uint32_t)w * (TI * WAVE) + c * WAVE + ln;
+            const bool valid = FULL || e < tl;
+            const uint32_t bin = valid ? pk[c] : (uint32_t)NB;
+            const uint64_t peers = match_peers<BITS>(bin, valid);
+            const uint32_t r = mask_rank(peers);
+            const uint32_t old = sh.wcnt[w][bin];
+            if (valid && r == 0) sh.wcnt[w][bin] = old + (uint32_t)__popcll(peers);
+            pk[c] = bin | ((old + r) << 8);
+        }
+        // the indices are loaded only now (they are not needed for the
+        // ranking); the barrier and the offset scan cover their latency
+#pragma unroll
+        for (int c = 0; c < TI; c++) {
+            const uint32_t e = (uint32_t)w * (TI * WAVE) + c * WAVE + ln;
+            if (IDX) ix[c] = (FULL || e < tl) ? isrc[ts + e] : 0u;
+        }
+        block_sync();
+        if (w == 0) {
+            const uint32_t l = ln;
+            uint32_t tot = 0;
+            if (l < (uint32_t)NB) {
+#pragma unroll
+                for (int i = 0; i < NW; i++) tot += sh.wcnt[i][l];
+            }
+            const uint32_t start = wave_incl_scan(tot) - tot;
+            if (l < (uint32_t)NB) {
+                uint32_t run = start;
+#pragma unroll
+                for (int i = 0; i < NW; i++) {
+                    sh.woff[i][l] = run;
+                    run += sh.wcnt[i][l];
+                }
+                const size_t e0 = (size_t)tb * NB;
+                const uint32_t rel = p.tpre[e0 + (size_t)l * ntl + t] - p.tpre[e0];
+                sh.dbase[l] = begin + rel - start;
+            }
+        }
+        block_sync();
+#pragma unroll
+        for (int c = 0; c < TI; c++) {
+            const uint32_t e = (uint32_t)w * (TI * WAVE) + c * WAVE + ln;
+            const uint32_t bin = pk[c] & 0xffu;
+            uint32_t pos = sh.woff[w][bin] + (pk[c] >> 8);
+            const bool valid = FULL || e < tl;
+            bad |= valid && pos >= tl;
+            pos &= TILE - 1;                   // stays inside the staging arrays
+            if (valid) {
+                stk[pos] = x[c];
+                if (IDX) sti[pos] = ix[c];
+                stb[pos] = (unsigned char)bin;
+            }
+        }
+        block_sync();
+#pragma nounroll
+        for (uint32_t i = tid; i < tl; i += NT) {
+            {
+                const uint32_t b = stb[i];
+                const uint32_t d = sh.dbase[b < (uint32_t)NB ? b : 0u] + i;
+                const bool ok = b < (uint32_t)NB && d - begin < len;   // inside the segment
+                bad |= !ok;
+                if (ok) {
+                    U *kd = (b & 1u) ? kfin : knext;
+                    kd[d] = stk[i];
+                    if (IDX) {
+                        uint32_t *id = (b & 1u) ? ifin : inext;
+                        id[d] = sti[i];
+                    }
+                }
+            }
+        }
+        if (bad) range_fail(p, 8);
+        block_sync();
+    }
+
+    static __device__ void scatter(const Params &p, unsigned char *arena, SharedTile &sh)
+    {
+        const uint32_t ntot = p.ctrl->tile_alloc < p.tilecap ? p.ctrl->tile_alloc : p.tilecap;
+        const uint32_t G = p.tiles_per_wg;
         uint64_t nbytes = 0;
         uint32_t cur = ~0u;
         Reg reg;
@@ -440,95 +561,11 @@ struct LevelK {
                     cur = sid;
                 }
                 load_reg(sh.sp, reg);
-                const uint32_t begin = sg->begin, len = sg->len, depth = sg->depth;
-                const uint32_t tb = sg->tile_base, ntl = sg->ntiles;
-                const U *src = src_keys<U>(p, depth) + begin;
-                const uint32_t *isrc = src_idx(p, depth) + begin;
-                U *knext = next_keys<U>(p, depth);
-                uint32_t *inext = next_idx(p, depth);
                 const uint32_t ts = t * TILE;
-                const uint32_t tl = (len - ts) < TILE ? (len - ts) : TILE;
+                const uint32_t tl = (sg->len - ts) < TILE ? (sg->len - ts) : TILE;
                 nbytes += 2ull * tl * (KB + C::IB);
-                if (lane() < (uint32_t)NBP) sh.wcnt[w][lane()] = 0;
-                wave_sync();
-                U x[TI];
-                uint32_t ix[TI];
-                uint32_t pk[TI];   // bin | rank inside the wave's bin << 8
-#pragma unroll
-                for (int c = 0; c < TI; c++) {
-                    const uint32_t e = (uint32_t)w * (TI * WAVE) + c * WAVE + lane();
-                    x[c] = e < tl ? src[ts + e] : (U)0;
-                    if (IDX) ix[c] = e < tl ? isrc[ts + e] : 0u;
-                }
-                // rank: running per-wave bin counter (one wave owns wcnt[w],
-                // its LDS accesses complete in program order) + rank among
-                // the lanes of this step with the same bin
-#pragma unroll
-                for (int c = 0; c < TI; c++) pk[c] = classify<U, S, LOG>((U)(x[c] ^ flip), sh.sp, reg);
-#pragma unroll
-                for (int c = 0; c < TI; c++) {
-                    const uint32_t e = (uint32_t)w * (TI * WAVE) + c * WAVE + lane();
-                    const bool valid = e < tl;
-                    const uint32_t bin = valid ? pk[c] : (uint32_t)NB;
-                    const uint64_t peers = match_peers<BITS>(bin, valid);
-                    const uint32_t r = mask_rank(peers);
-                    const uint32_t old = sh.wcnt[w][bin];
-                    if (valid && r == 0) sh.wcnt[w][bin] = old + (uint32_t)__popcll(peers);
-                    pk[c] = bin | ((old + r) << 8);
-                }
-                block_sync();
-                if (w == 0) {
-                    const uint32_t l = lane();
-                    uint32_t tot = 0;
-                    if (l < (uint32_t)NB) {
-#pragma unroll
-                        for (int i = 0; i < NW; i++) tot += sh.wcnt[i][l];
-                    }
-                    const uint32_t start = wave_incl_scan(tot) - tot;
-                    if (l < (uint32_t)NB) {
-                        uint32_t run = start;
-#pragma unroll
-                        for (int i = 0; i < NW; i++) {
-                            sh.woff[i][l] = run;
-                            run += sh.wcnt[i][l];
-                        }
-                        const size_t e0 = (size_t)tb * NB;
-                        const uint32_t rel = p.tpre[e0 + (size_t)l * ntl + t] - p.tpre[e0];
-                        sh.dbase[l] = begin + rel - start;
-                    }
-                }
-                block_sync();
-#pragma unroll
-                for (int c = 0; c < TI; c++) {
-                    const uint32_t e = (uint32_t)w * (TI * WAVE) + c * WAVE + lane();
-                    if (e < tl) {
-                        const uint32_t bin = pk[c] & 0xffu;
-                        const uint32_t pos = sh.woff[w][bin] + (pk[c] >> 8);
-                        if (pos < tl) {
-                            stk[pos] = x[c];
-                            if (IDX) sti[pos] = ix[c];
-                            stb[pos] = (unsigned char)bin;
-                        } else {
-                            range_fail(p, 8);
-                        }
-                    }
-                }
-                block_sync();
-                for (uint32_t i = threadIdx.x; i < tl; i += NT) {
-                    const uint32_t b = stb[i];
-                    const uint32_t d = b < (uint32_t)NB ? sh.dbase[b] + i : 0xffffffffu;
-                    if (d - begin >= len) {        // outside the segment (also catches b >= NB)
-                        range_fail(p, 9);
-                        continue;
-                    }
-                    U *kd = (b & 1u) ? kfin : knext;
-                    kd[d] = stk[i];
-                    if (IDX) {
-                        uint32_t *id = (b & 1u) ? ifin : inext;
-                        id[d] = sti[i];
-                    }
-                }
-                block_sync();
+                if (tl == TILE) scatter_tile<true>(p, arena, sh, sg, t, reg);
+                else scatter_tile<false>(p, arena, sh, sg, t, reg);
             }
         }
         if (threadIdx.x == 0 && nbytes) {
