@@ -206,6 +206,54 @@ def bench_batched(torch, np, wl, TurtleSort, fill_keys, reps):
     }
 
 
+def bench_small_configs(torch, np, wl, TurtleSort, fill_keys, fill_iota, reps=20):
+    """config 1 (1,048,576 int32 keys in [0,32767] + iota, device-resident
+    sort: a latency case that fits the 256 MB Infinity Cache) and config 3
+    (MultiSort 65,536 arrays x 1024 int32, one launch)"""
+    out = {}
+    n = wl.C1_N
+    s = TurtleSort(np.int32, n)
+    kin = torch.empty(n, dtype=torch.int32, device="cuda")
+    fill_keys(kin, wl.SEED_C1, wl.SHIFT_15BIT)
+    iin = torch.empty_like(kin)
+    fill_iota(iin)
+    kout, iout = torch.empty_like(kin), torch.empty_like(iin)
+    s.sort_device(kin, kout, iin, iout)
+    s.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        s.sort_device(kin, kout, iin, iout)
+    s.synchronize()
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / reps
+    ref_k, ref_i = torch.sort(kin.to(torch.int64), stable=True)
+    out["c1_plain_1M_dup15"] = {"ms_per_sort": round(t * 1e3, 4), "Gkeys_s": round(n / t / 1e9, 4),
+                                "valid": bool(torch.equal(kout.to(torch.int64), ref_k))
+                                and bool(torch.equal(iout.to(torch.int64), ref_i))}
+    s.close()
+    num, arr = wl.C3_ARRAYS, wl.C3_SIZE
+    s = TurtleSort(np.int32, num * arr, track_index=False)
+    kin = torch.empty(num * arr, dtype=torch.int32, device="cuda")
+    fill_keys(kin, wl.SEED_C3, wl.SHIFT_31BIT)
+    kout = torch.empty_like(kin)
+    s.multi_sort_device(arr, kin, kout)
+    s.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(reps):
+        s.multi_sort_device(arr, kin, kout)
+    ev1.record()
+    torch.cuda.synchronize()
+    t = ev0.elapsed_time(ev1) / 1e3 / reps
+    ref, _ = torch.sort(kin.view(num, arr), dim=1)
+    out["c3_multisort_65536x1024"] = {"ms_per_launch": round(t * 1e3, 4), "Gkeys_s": round(num * arr / t / 1e9, 3),
+                                      "hbm_frac": round(2 * 4 * num * arr / t / 1e9 / HBM_PEAK_GBS, 4),
+                                      "valid": bool(torch.equal(kout.view(num, arr), ref))}
+    s.close()
+    return out
+
+
 def bench_serving(seconds, rate):
     """open-loop request stream through the server (examples/serve_load)"""
     import subprocess
@@ -352,6 +400,9 @@ def main():
                               "Gkeys_s": round(n / min(api_s) / 1e9, 4), "seconds": [round(t, 4) for t in api_s],
                               "valid": bool((k[1:] >= k[:-1]).all() and (hk[x] == k).all())}
     del hk, hi, k, x
+
+    if not args.no_variants:
+        out["small_configs"] = bench_small_configs(torch, np, wl, TurtleSort, fill_keys, fill_iota)
 
     if not args.no_batched:
         del kin, kout, iin, iout

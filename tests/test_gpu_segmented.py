@@ -103,3 +103,31 @@ def test_segmented_graph_replays(orc, torch_dev):
         torch.cuda.synchronize()
     g.close()
     sorter.close()
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64, np.uint8, np.int16])
+def test_lds_size_classes(orc, torch_dev, dtype):
+    """Segments at every LDS-sort size-class boundary (merge classes of
+    64..512 keys, radix classes of 1..8 waves of 1024 or 512 slots), each
+    sorted in LDS directly: keys equal the oracle's, the index is the stable
+    order.  Two key ranges: full (several radix passes) and 8 values (one)."""
+    torch = torch_dev
+    lens = [1, 2, 63, 64, 65, 511, 512, 513, 1023, 1024, 1025, 1500, 2047, 2048, 2049, 3071, 3072, 3073,
+            4095, 4096, 4097, 5000, 6143, 6144, 6145, 7000, 8191, 8192]
+    if np.dtype(dtype).itemsize == 8:
+        lens = [x for x in lens if x <= 4096]
+    off = wl.segment_offsets(np.array(lens, dtype=np.int64))
+    n = int(off[-1])
+    sorter = TurtleSort(dtype, n)
+    for shift in (0, 61):
+        print(f"case dtype={np.dtype(dtype)} shift={shift}", flush=True)
+        keys = wl.gen_keys(dtype, n, 91 + shift, shift)
+        kin = h2d(torch, keys.view(np.int16) if dtype == np.uint16 else keys)
+        iin = torch.arange(n, dtype=torch.int32, device="cuda")
+        kout = torch.empty_like(kin)
+        iout = torch.empty_like(iin)
+        sorter.segmented_sort_device(kin, kout, off, iin, iout)
+        st = sorter.synchronize()
+        assert st["error"] == 0 and st["segments"] == 0     # every segment went straight to the LDS sort
+        check_segments(orc, keys, off, d2h(torch, kout), d2h(torch, iout))
+    sorter.close()

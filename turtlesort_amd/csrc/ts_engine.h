@@ -28,7 +28,7 @@ enum : int {
 constexpr int MAX_SPLIT = 31;
 constexpr int MAX_BINS = 2 * MAX_SPLIT + 1;
 constexpr int MAX_LEVELS = 64;
-constexpr int LDS_CLASSES = 16;        // LDS-sort size classes: n_eff = 2^c
+constexpr int LDS_CLASSES = 20;        // LDS-sort size classes
 constexpr uint32_t SCAN_CHUNK = 4096;  // tile-count entries per workgroup of the level scan
 
 // Device control block (one per handle), cleared by the reset kernel before
@@ -52,6 +52,7 @@ struct Ctrl {
     uint32_t cls_wg[LDS_CLASSES + 1];   // first workgroup step of the class; [LDS_CLASSES] = total
     uint32_t pad2[3];
 };
+static_assert(sizeof(Ctrl) % 4 == 0, "Ctrl is cleared word by word");
 
 // Segments of level L are the ids [sb, se) of the segment table.
 struct Level {

@@ -71,8 +71,9 @@ __global__ void k_level(Params p, uint32_t L)
 }
 
 // LDS-sort plan: first slot and first workgroup step of every size class
-// (class c: n_eff = 2^c, cap / 2^c tasks per workgroup step)
-__global__ void k_ldsplan(Params p, uint32_t cmin, uint32_t cmax, uint32_t cap)
+// (class c < crad: cap / 2^c tasks per workgroup step; c >= crad: tasks of
+// c - crad + 1 whole waves, nw / (c - crad + 1) per step)
+__global__ void k_ldsplan(Params p, uint32_t crad, uint32_t nw, uint32_t cap)
 {
     if (threadIdx.x == 0) {
         Ctrl *c = p.ctrl;
@@ -83,10 +84,9 @@ __global__ void k_ldsplan(Params p, uint32_t cmin, uint32_t cmax, uint32_t cap)
             c->cls_cur[k] = 0;
             const uint32_t n = c->cls_cnt[k];
             off += n;
-            if (k >= cmin && k <= cmax && n) {
-                const uint32_t P = cap >> k;
-                wg += (n + P - 1) / P;
-            }
+            uint32_t P = k < crad ? cap >> k : (k - crad + 1 <= nw ? nw / (k - crad + 1) : 1u);
+            if (P == 0) P = 1;
+            if (n) wg += (n + P - 1) / P;
         }
         c->cls_wg[LDS_CLASSES] = wg;
     }

@@ -63,14 +63,19 @@ struct Cfg {
     static constexpr uint32_t SAMPLE = 1024;
     static constexpr uint32_t PP = CAP / SAMPLE;          // segments per pivot workgroup step
     static constexpr uint32_t IB = IDX ? 4 : 0;
-    // LDS-sort size classes c (n_eff = 2^c) from 64 keys up to CAP
+    // LDS-sort size classes (see LevelK::lds_class): merge classes
+    // [CMIN, CRAD) of n_eff = 2^c keys, radix classes [CRAD, CLAST] of
+    // c - CRAD + 1 whole waves of WS slots
     static constexpr int CMIN = 6, CMAX = ilog2(CAP);
+    static constexpr uint32_t WS = (uint32_t)SI * WAVE;
+    static constexpr int CRAD = ilog2(WS);
+    static constexpr int CLAST = CRAD + NW - 1;
     static constexpr uint32_t SORT_KEY_BYTES = (padded_len<SI>(CAP) * KB + 15) & ~15u;
     static constexpr uint32_t SORT_BYTES = SORT_KEY_BYTES + (IDX ? padded_len<SI>(CAP) * 4 : 0);
     static constexpr uint32_t SCAT_BYTES = TILE * KB + TILE * IB + TILE;
     static_assert((S + 1) == (1 << LOG), "S + 1 must be a power of two");
     static_assert(S <= MAX_SPLIT && NBP <= WAVE, "too many splitters");
-    static_assert(CMAX < LDS_CLASSES && (1 << CMIN) >= SI, "size classes");
+    static_assert(CLAST < LDS_CLASSES && (1 << CMIN) >= SI && CMIN < CRAD && WS * NW == CAP, "size classes");
 };
 
 // ---- buffers by depth -----------------------------------------------------------
@@ -167,20 +172,11 @@ __device__ __forceinline__ uint32_t classify(U x, const Split<U, S> &sp, const U
     return eq ? 2 * lt + 1 : 2 * lt;
 }
 
-// LDS-sort size class of a task of len keys: n_eff = 2^class
-// (clamped to the class table; a task never exceeds the LDS capacity)
-template <int CMIN>
-__device__ __forceinline__ uint32_t lds_class(uint32_t len)
-{
-    const uint32_t c = len <= 1 ? 0u : (uint32_t)(32 - __clz(len - 1));
-    return c < (uint32_t)CMIN ? (uint32_t)CMIN : (c < (uint32_t)LDS_CLASSES ? c : (uint32_t)LDS_CLASSES - 1);
-}
-
 // ---- set-up kernels ------------------------------------------------------------------
 __global__ void k_reset(Ctrl *c);
 __global__ void k_roots(Params p, const uint32_t *off, uint32_t nseg, uint32_t single_len);
 __global__ void k_level(Params p, uint32_t L);
-__global__ void k_ldsplan(Params p, uint32_t cmin, uint32_t cmax, uint32_t cap);
+__global__ void k_ldsplan(Params p, uint32_t crad, uint32_t nw, uint32_t cap);
 
 template <int KB, bool IDX, int S>
 struct LevelK {
@@ -640,7 +636,7 @@ struct LevelK {
         if (threadIdx.x < (uint32_t)LDS_CLASSES) hist[threadIdx.x] = 0;
         block_sync();
         for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < ntask; k += gridDim.x * blockDim.x)
-            atomicAdd(&hist[lds_class<C::CMIN>(p.ldst[k].len)], 1u);
+            atomicAdd(&hist[lds_class(p.ldst[k].len)], 1u);
         block_sync();
         if (threadIdx.x < (uint32_t)LDS_CLASSES && hist[threadIdx.x]) atomic_add(&p.ctrl->cls_cnt[threadIdx.x], hist[threadIdx.x]);
     }
@@ -657,7 +653,7 @@ struct LevelK {
             const uint32_t k = k0 + threadIdx.x;
             uint32_t c = 0, lr = 0;
             if (k < ntask) {
-                c = lds_class<C::CMIN>(p.ldst[k].len);
+                c = lds_class(p.ldst[k].len);
                 lr = atomicAdd(&hist[c], 1u);
             }
             block_sync();
@@ -673,18 +669,33 @@ struct LevelK {
         }
     }
 
-    // ---------------------------------------------------------------- LDS radix
-    // Tasks of class c >= CRAD fill whole waves: wave w's WS slots belong to
-    // task w >> (c - CRAD).  Their keys span a narrow range (they lie
-    // between two splitters of their parent), so a stable LSD radix sort of
-    // key - min over the range's bits takes a few passes of RW-bit digits,
-    // each pass ranking the keys with wave64 match masks and per-wave digit
-    // counters; padding slots take the largest digit in every pass and so
-    // stay behind the task's keys.
+    // ---------------------------------------------------------------- LDS sorts
+    // Size class of a leaf task of len keys (WS = the slots of one wave):
+    // len <= WS/2: class c = max(CMIN, ceil(log2 len)), merge-sorted, CAP >> c
+    // tasks per workgroup step in power-of-two regions; longer: class
+    // CRAD + kw - 1 with kw = ceil(len / WS) whole waves, radix-sorted,
+    // NW / kw tasks per step.
+    static constexpr uint32_t WS = C::WS;
+    static constexpr uint32_t CRAD = (uint32_t)C::CRAD, CLAST = (uint32_t)C::CLAST;
+
+    static __device__ __forceinline__ uint32_t lds_class(uint32_t len)
+    {
+        if (len <= WS / 2) {
+            const uint32_t c = len <= 1 ? 0u : (uint32_t)(32 - __clz(len - 1));
+            return c < (uint32_t)C::CMIN ? (uint32_t)C::CMIN : c;
+        }
+        const uint32_t kw = (len + WS - 1) / WS;
+        return kw <= (uint32_t)NW ? CRAD + kw - 1 : CLAST;   // len <= CAP: kw <= NW
+    }
+
+    // Radix classes: a task's keys span a narrow range (they lie between two
+    // splitters of their parent), so a stable LSD radix sort of key - min
+    // over the range's bits takes a few passes of RW-bit digits, each pass
+    // ranking the keys with wave64 match masks and per-wave digit counters;
+    // padding slots take the largest digit in every pass and so stay behind
+    // the task's keys.
     static constexpr int RW = 9;
     static constexpr uint32_t NBIN = 1u << RW;
-    static constexpr uint32_t WS = (uint32_t)SI * WAVE;
-    static constexpr uint32_t CRAD = (uint32_t)ilog2(WS);
     static constexpr uint32_t MAXPASS = (8 * KB + RW - 1) / RW;   // covers any key range
     static constexpr uint32_t EPT = NW * NBIN / NT;   // scan entries per thread
 
@@ -715,26 +726,42 @@ struct LevelK {
         return x;
     }
 
-    static __device__ void radix(U *sk, uint32_t *si, const TaskRec *rec, uint32_t c, uint32_t tid, RadixShared &rs)
+    // One workgroup step of a radix class: nk tasks of kw waves each; wave w
+    // holds items i at task position (w % kw)*WS + i*WAVE + lane, read from
+    // global memory straight into registers.  The step's LDS arrays are in
+    // slot order (task t at slots [t*kw*WS, (t+1)*kw*WS)); on return they
+    // hold every task's keys sorted at the start of its slots.
+    // floor(x / kw) = (x * (2^32/kw + 1)) >> 32 in 64-bit arithmetic, exact
+    // for x < 2^15 and kw <= 8 (all slots and scan entries are below 2^13).
+    static __device__ void radix_step(const Params &p, U *sk, uint32_t *si, const TaskRec *rec, uint32_t kw,
+                                      uint32_t nk, uint32_t tid, RadixShared &rs)
     {
-        const uint32_t w = tid / WAVE, l = tid % WAVE;
-        const uint32_t wl = c - CRAD;                     // log2(waves per task)
-        const uint32_t q = w >> wl;
-        const uint32_t j0 = (w & ((1u << wl) - 1)) * WS;  // first slot of this wave inside its task
-        const uint32_t len = rec[q].len;
-        const uint32_t s0 = w * WS + l;                   // slot of item i: s0 + i*WAVE
-        // pidx(s0 + i*WAVE) = pidx(s0) + i*(WAVE + WAVE/SI): one base, constant offsets
-        constexpr uint32_t ISTR = WAVE + WAVE / SI;
-        U *const rk = sk + pidx<SI>(s0);
-        uint32_t *const ri = si + pidx<SI>(s0);
-        if (tid == 0) rs.maxbits = 0;
+        const U flip = (U)p.flip;
+        const uint64_t mkw = (1ull << 32) / kw + 1ull;
+        const uint32_t w = __builtin_amdgcn_readfirstlane(tid / WAVE), l = tid % WAVE;
+        const uint32_t q = (uint32_t)((w * mkw) >> 32);   // task of this wave
+        const bool act = q < nk;                          // waves past the step's tasks idle
+        const uint32_t j0 = (w - q * kw) * WS + l;        // task position of item i: j0 + i*WAVE
+        const TaskRec tr = act ? rec[q] : TaskRec{0, 0, 0};
+        U *const rk = sk + w * WS + l;                    // slot of item i: rk[i*WAVE]
+        uint32_t *const ri = si + w * WS + l;
+        U k[SI];
+        uint32_t v[SI];
         U mn = key_max<U>(), mx = 0;
+        {
+            const U *src = src_keys<U>(p, tr.depth) + tr.begin;
+            const uint32_t *isrc = src_idx(p, tr.depth) + tr.begin;
 #pragma unroll
-        for (int i = 0; i < SI; i++) {
-            if (j0 + (uint32_t)i * WAVE + l < len) {
-                const U x = rk[i * ISTR];
-                mn = x < mn ? x : mn;
-                mx = x > mx ? x : mx;
+            for (int i = 0; i < SI; i++) {
+                const uint32_t j = j0 + (uint32_t)i * WAVE;
+                k[i] = key_max<U>();
+                v[i] = 0u;
+                if (j < tr.len) {
+                    k[i] = (U)(src[j] ^ flip);
+                    if (IDX) v[i] = isrc[j];
+                    mn = k[i] < mn ? k[i] : mn;
+                    mx = k[i] > mx ? k[i] : mx;
+                }
             }
         }
         mn = wave_min(mn);
@@ -743,55 +770,60 @@ struct LevelK {
             rs.wmin[w] = mn;
             rs.wmax[w] = mx;
         }
+        if (tid == 0) rs.maxbits = 0;
         block_sync();
-        if (tid < (uint32_t)(NW >> wl)) {
+        if (tid < nk) {
             U a = key_max<U>(), b = 0;
-            for (uint32_t k = 0; k < (1u << wl); k++) {
-                const U x = rs.wmin[(tid << wl) + k], y = rs.wmax[(tid << wl) + k];
-                a = x < a ? x : a;
-                b = y > b ? y : b;
+            for (uint32_t x = tid * kw; x < (tid + 1) * kw; x++) {
+                a = rs.wmin[x] < a ? rs.wmin[x] : a;
+                b = rs.wmax[x] > b ? rs.wmax[x] : b;
             }
             rs.tmin[tid] = a;
             const uint64_t span = b > a ? (uint64_t)(U)(b - a) : 0ull;
-            const uint32_t bits = span ? (uint32_t)(64 - __clzll(span)) : 0u;
-            atomicMax(&rs.maxbits, bits);
+            atomicMax(&rs.maxbits, span ? (uint32_t)(64 - __clzll(span)) : 0u);
         }
         block_sync();
         uint32_t npass = (rs.maxbits + RW - 1) / RW;
         npass = npass < MAXPASS ? npass : MAXPASS;
-        const U tmin = rs.tmin[q];
+        const U tmin = act ? rs.tmin[q] : (U)0;
+        const uint32_t E = nk * kw * NBIN;                // scan entries of the step
         for (uint32_t pass = 0; pass < npass; pass++) {
             const uint32_t sh = pass * RW;
-            U k[SI];
-            uint32_t v[SI], dr[SI];
+            if (pass > 0 && act) {
 #pragma unroll
-            for (int i = 0; i < SI; i++) {
-                k[i] = rk[i * ISTR];
-                if (IDX) v[i] = ri[i * ISTR];
+                for (int i = 0; i < SI; i++) {
+                    k[i] = rk[i * WAVE];
+                    if (IDX) v[i] = ri[i * WAVE];
+                }
             }
-            for (uint32_t d = l; d < NBIN; d += WAVE) rs.cnt[w][d] = 0;
-            wave_sync();
+            uint32_t dr[SI];
+            if (act) {
+                for (uint32_t d = l; d < NBIN; d += WAVE) rs.cnt[w][d] = 0;
+                wave_sync();
 #pragma unroll
-            for (int i = 0; i < SI; i++) {
-                const bool real = j0 + (uint32_t)i * WAVE + l < len;
-                const uint32_t d = real ? (uint32_t)((U)(k[i] - tmin) >> sh) & (NBIN - 1) : NBIN - 1;
-                const uint64_t peers = match_peers<RW>(d, true);
-                const uint32_t r = mask_rank(peers);
-                const uint32_t old = rs.cnt[w][d];
-                if (r == 0) rs.cnt[w][d] = (uint16_t)(old + (uint32_t)__popcll(peers));
-                dr[i] = d | ((old + r) << 16);
+                for (int i = 0; i < SI; i++) {
+                    // after a pass the task's keys are at its first len positions
+                    const bool real = j0 + (uint32_t)i * WAVE < tr.len;
+                    const uint32_t d = real ? (uint32_t)((U)(k[i] - tmin) >> sh) & (NBIN - 1) : NBIN - 1;
+                    const uint64_t peers = match_peers<RW>(d, true);
+                    const uint32_t r = mask_rank(peers);
+                    const uint32_t old = rs.cnt[w][d];
+                    if (r == 0) rs.cnt[w][d] = (uint16_t)(old + (uint32_t)__popcll(peers));
+                    dr[i] = d | ((old + r) << 16);
+                }
             }
             block_sync();
             // exclusive scan over (task, digit, wave of the task): the first
-            // slot of every (wave, digit) run in the step
+            // slot of every (wave, digit) run of the step
             {
-                uint32_t e0 = tid * EPT, cv[EPT], loc = 0;
+                const uint32_t e0 = tid * EPT;
+                uint32_t cv[EPT], loc = 0;
 #pragma unroll
                 for (int k2 = 0; k2 < (int)EPT; k2++) {
                     const uint32_t e = e0 + k2;
-                    const uint32_t ww = ((e >> (wl + RW)) << wl) + (e & ((1u << wl) - 1));
-                    const uint32_t d = (e >> wl) & (NBIN - 1);
-                    cv[k2] = rs.cnt[ww][d];
+                    const uint32_t td = (uint32_t)((e * mkw) >> 32);   // task*NBIN + digit
+                    const uint32_t ww = (td >> RW) * kw + (e - td * kw);
+                    cv[k2] = e < E ? (uint32_t)rs.cnt[ww < NW ? ww : 0][td & (NBIN - 1)] : 0u;
                     loc += cv[k2];
                 }
                 uint32_t tot;
@@ -799,27 +831,38 @@ struct LevelK {
 #pragma unroll
                 for (int k2 = 0; k2 < (int)EPT; k2++) {
                     const uint32_t e = e0 + k2;
-                    const uint32_t ww = ((e >> (wl + RW)) << wl) + (e & ((1u << wl) - 1));
-                    const uint32_t d = (e >> wl) & (NBIN - 1);
-                    rs.cnt[ww][d] = (uint16_t)run;
+                    const uint32_t td = (uint32_t)((e * mkw) >> 32);
+                    const uint32_t ww = (td >> RW) * kw + (e - td * kw);
+                    if (e < E) rs.cnt[ww][td & (NBIN - 1)] = (uint16_t)run;
                     run += cv[k2];
                 }
             }
             block_sync();
+            if (act) {
 #pragma unroll
-            for (int i = 0; i < SI; i++) {
-                const uint32_t pos = rs.cnt[w][dr[i] & 0xffffu] + (dr[i] >> 16);
-                if (pos < CAP) {
-                    sk[pidx<SI>(pos)] = k[i];
-                    if (IDX) si[pidx<SI>(pos)] = v[i];
+                for (int i = 0; i < SI; i++) {
+                    const uint32_t pos = (rs.cnt[w][dr[i] & 0xffffu] + (dr[i] >> 16)) & (CAP - 1);
+                    sk[pos] = k[i];
+                    if (IDX) si[pos] = v[i];
+                }
+            }
+            block_sync();
+        }
+        if (npass == 0) {
+            // every task's keys are equal: they stay where they are
+            if (act) {
+#pragma unroll
+                for (int i = 0; i < SI; i++) {
+                    rk[i * WAVE] = k[i];
+                    if (IDX) ri[i * WAVE] = v[i];
                 }
             }
             block_sync();
         }
     }
 
-    // RAD: the workgroup steps of classes [CRAD, CMAX] (radix), else those
-    // of classes [CMIN, CRAD) (merge sort); two launches, disjoint tasks
+    // RAD: the workgroup steps of the radix classes [CRAD, CLAST], else those
+    // of the merge classes [CMIN, CRAD); two launches, disjoint tasks
     template <bool RAD>
     static __device__ void lds(const Params &p, unsigned char *arena, TaskRec *rec, RadixShared &rs,
                                unsigned long long *acc)
@@ -829,9 +872,8 @@ struct LevelK {
         const U flip = (U)p.flip;
         U *dst = (U *)p.kout;
         const Ctrl *cc = p.ctrl;
-        const uint32_t cr = CRAD < (uint32_t)C::CMAX + 1 ? CRAD : (uint32_t)C::CMAX + 1;
-        const uint32_t vbeg = RAD ? cc->cls_wg[cr] : 0u;
-        const uint32_t vend = RAD ? cc->cls_wg[LDS_CLASSES] : cc->cls_wg[cr];
+        const uint32_t c0 = RAD ? CRAD : (uint32_t)C::CMIN, clast = RAD ? CLAST : CRAD - 1;
+        const uint32_t vbeg = cc->cls_wg[c0], vend = cc->cls_wg[clast + 1];
         uint64_t nbytes = 0, ntasks = 0;
         if (threadIdx.x == 0) acc[0] = acc[1] = 0;
         for (uint32_t v = vbeg + blockIdx.x; v < vend; v += gridDim.x) {
@@ -841,9 +883,11 @@ struct LevelK {
             // spills them
             uint32_t tid = threadIdx.x;
             asm volatile("" : "+v"(tid));
-            uint32_t c = C::CMIN;
-            while (c < (uint32_t)C::CMAX && v >= cc->cls_wg[c + 1]) c++;
-            const uint32_t n_eff = 1u << c, P = CAP >> c;
+            uint32_t c = c0;
+            while (c < clast && v >= cc->cls_wg[c + 1]) c++;
+            const uint32_t kw = RAD ? c - CRAD + 1 : 0u;
+            const uint32_t region = RAD ? kw * WS : 1u << c;     // slots per task
+            const uint32_t P = RAD ? (uint32_t)NW / kw : CAP >> c;
             const uint32_t first = (v - cc->cls_wg[c]) * P;
             const uint32_t cnt = cc->cls_cnt[c];
             const uint32_t nk = first < cnt ? ((cnt - first) < P ? (cnt - first) : P) : 0u;
@@ -856,7 +900,7 @@ struct LevelK {
                         const LdsTask tk = p.ldst[k];
                         r = TaskRec{tk.begin, tk.len, tk.depth};
                     }
-                    if (!span_ok(p, r.begin, r.len) || r.len > n_eff) {
+                    if (!span_ok(p, r.begin, r.len) || r.len > region) {
                         range_fail(p, 1);
                         r.len = 0;
                     }
@@ -866,53 +910,66 @@ struct LevelK {
                 rec[q] = r;
             }
             block_sync();
-            // the global loads of a batch are issued before its LDS stores,
-            // so they are in flight together (batches of 8 bound the
-            // registers they hold)
-            for (int r0 = 0; r0 < SI; r0 += 8) {
-                U kv[8];
-                uint32_t iv[8];
+            if constexpr (RAD) {
+                radix_step(p, sk, si, rec, kw, nk, tid, rs);
+                // write-out: task t's sorted keys are at slots [t*region, t*region + len)
+                uint32_t tid2 = threadIdx.x;
+                asm volatile("" : "+v"(tid2));
+                const uint64_t mkw = (1ull << 32) / kw + 1ull;   // floor(x / kw) = (x * mkw) >> 32
 #pragma unroll
-                for (int r = 0; r < 8; r++) {
-                    const uint32_t i = tid + (uint32_t)(r0 + r) * NT;
-                    const uint32_t q = i >> c, j = i & (n_eff - 1);
-                    const TaskRec tr = rec[q];
-                    kv[r] = key_max<U>();
-                    iv[r] = 0u;
-                    if (j < tr.len) {
-                        kv[r] = (U)(src_keys<U>(p, tr.depth)[tr.begin + j] ^ flip);
-                        if (IDX) iv[r] = src_idx(p, tr.depth)[tr.begin + j];
+                for (int r = 0; r < SI; r++) {
+                    const uint32_t s = tid2 + (uint32_t)r * NT;
+                    const uint32_t t = (uint32_t)(((s / WS) * mkw) >> 32);
+                    const uint32_t j = s - t * region;
+                    if (t < nk) {
+                        const TaskRec tr = rec[t];
+                        if (j < tr.len) {
+                            dst[tr.begin + j] = (U)(sk[s] ^ flip);
+                            if (IDX) p.iout[tr.begin + j] = si[s];
+                        }
                     }
                 }
-                // pidx(tid + r*NT) = pidx(tid) + r*(NT + NT/SI)
-                U *const mk = sk + pidx<SI>(tid) + (uint32_t)r0 * (NT + NT / SI);
-                uint32_t *const mi = si + pidx<SI>(tid) + (uint32_t)r0 * (NT + NT / SI);
-#pragma unroll
-                for (int r = 0; r < 8; r++) {
-                    mk[r * (NT + NT / SI)] = kv[r];
-                    if (IDX) mi[r * (NT + NT / SI)] = iv[r];
-                }
-            }
-            block_sync();
-            if constexpr (RAD) {
-                uint32_t tid3 = threadIdx.x;
-                asm volatile("" : "+v"(tid3));
-                radix(sk, si, rec, c, tid3, rs);
             } else {
+                const uint32_t n_eff = region;
+                // the global loads of a batch are issued before its LDS
+                // stores, so they are in flight together (batches of 8 bound
+                // the registers they hold)
+                for (int r0 = 0; r0 < SI; r0 += 8) {
+                    U kv[8];
+                    uint32_t iv[8];
+#pragma unroll
+                    for (int r = 0; r < 8; r++) {
+                        const uint32_t i = tid + (uint32_t)(r0 + r) * NT;
+                        const uint32_t q = i >> c, j = i & (n_eff - 1);
+                        const TaskRec tr = rec[q];
+                        kv[r] = key_max<U>();
+                        iv[r] = 0u;
+                        if (j < tr.len) {
+                            kv[r] = (U)(src_keys<U>(p, tr.depth)[tr.begin + j] ^ flip);
+                            if (IDX) iv[r] = src_idx(p, tr.depth)[tr.begin + j];
+                        }
+                    }
+                    // pidx(tid + r*NT) = pidx(tid) + r*(NT + NT/SI)
+                    U *const mk = sk + pidx<SI>(tid) + (uint32_t)r0 * (NT + NT / SI);
+                    uint32_t *const mi = si + pidx<SI>(tid) + (uint32_t)r0 * (NT + NT / SI);
+#pragma unroll
+                    for (int r = 0; r < 8; r++) {
+                        mk[r * (NT + NT / SI)] = kv[r];
+                        if (IDX) mi[r * (NT + NT / SI)] = iv[r];
+                    }
+                }
+                block_sync();
                 block_sort<U, IDX, NT, SI>(sk, si, CAP, n_eff);
-            }
-            {
                 // a fresh opaque copy of the thread index: the slot numbers
                 // below are recomputed here instead of being kept live (in
                 // spilled registers) from the fill through the sort
                 uint32_t tid2 = threadIdx.x;
                 asm volatile("" : "+v"(tid2));
-                const uint32_t tid = tid2;
-                const U *const mk = sk + pidx<SI>(tid);   // + r*(NT + NT/SI) for tid + r*NT
-                const uint32_t *const mi = si + pidx<SI>(tid);
+                const U *const mk = sk + pidx<SI>(tid2);   // + r*(NT + NT/SI) for tid + r*NT
+                const uint32_t *const mi = si + pidx<SI>(tid2);
 #pragma unroll
                 for (int r = 0; r < SI; r++) {
-                    const uint32_t i = tid + (uint32_t)r * NT;
+                    const uint32_t i = tid2 + (uint32_t)r * NT;
                     const uint32_t q = i >> c, j = i & (n_eff - 1);
                     const TaskRec tr = rec[q];
                     if (j < tr.len) {
@@ -1064,7 +1121,7 @@ hipError_t run_levels(const Params &p, int cus, int wg_per_cu, const uint32_t *d
     }
     if (last) {
         TS_LAUNCH(K_LDSPREP, (k_ldscount<KB, IDX, S>), dim3(g_small), dim3(256), 0, st, p);
-        TS_LAUNCH(K_LDSPREP, k_ldsplan, dim3(1), dim3(64), 0, st, p, (uint32_t)C::CMIN, (uint32_t)C::CMAX, C::CAP);
+        TS_LAUNCH(K_LDSPREP, k_ldsplan, dim3(1), dim3(64), 0, st, p, (uint32_t)C::CRAD, (uint32_t)C::NW, C::CAP);
         TS_LAUNCH(K_LDSPREP, (k_ldsbucket<KB, IDX, S>), dim3(g_small), dim3(256), 0, st, p);
         TS_LAUNCH(K_LDS, (k_lds<KB, IDX, S, false>), dim3(g_lds), dim3(512), 0, st, p);
         TS_LAUNCH(K_LDS, (k_lds<KB, IDX, S, true>), dim3(g_lds), dim3(512), 0, st, p);
