@@ -119,6 +119,7 @@ def test_lds_size_classes(orc, torch_dev, dtype):
     off = wl.segment_offsets(np.array(lens, dtype=np.int64))
     n = int(off[-1])
     sorter = TurtleSort(dtype, n)
+    sorter.set_tuning(lds_cutoff=4096 if np.dtype(dtype).itemsize == 8 else 8192)
     for shift in (0, 61):
         print(f"case dtype={np.dtype(dtype)} shift={shift}", flush=True)
         keys = wl.gen_keys(dtype, n, 91 + shift, shift)
