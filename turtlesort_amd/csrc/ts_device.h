@@ -68,22 +68,23 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
 
 // Lanes whose value v (< 2^BITS, or invalid) equals this lane's: AND of the
 // per-bit ballots, restricted to valid lanes.  Per bit: s = bit b of v
-// sign-extended (0 or ~0), bb = ballot of that bit; the lanes that agree
-// with this one are ~(s ^ bb), a 3-input bitwise function of (m, s, bb) per
-// 32-bit half (one v_bitop3 each on gfx950).
+// sign-extended (0 or ~0), bb = ballot of that bit; the lanes that differ
+// from this one in that bit are s ^ bb, accumulated with OR, a 3-input
+// bitwise function per 32-bit half (one v_bitop3 each on gfx950).
 template <int BITS>
 __device__ __forceinline__ uint64_t match_peers(uint32_t v, bool valid)
 {
-    const uint64_t vm = ballot(valid);
-    uint32_t lo = (uint32_t)vm, hi = (uint32_t)(vm >> 32);
+    // dl/dh collect the lanes that differ from this one in some bit
+    uint32_t dl = 0, dh = 0;
 #pragma unroll
     for (int b = 0; b < BITS; b++) {
-        const uint32_t s = (uint32_t)(((int32_t)(v << (31 - b))) >> 31);
-        const uint64_t bb = ballot(s != 0u);
-        lo &= ~(s ^ (uint32_t)bb);
-        hi &= ~(s ^ (uint32_t)(bb >> 32));
+        const int32_t si = __builtin_amdgcn_sbfe((int32_t)v, b, 1);   // bit b, sign-extended
+        const uint64_t bb = ballot(si < 0);
+        dl |= (uint32_t)si ^ (uint32_t)bb;
+        dh |= (uint32_t)si ^ (uint32_t)(bb >> 32);
     }
-    return ((uint64_t)hi << 32) | lo;
+    const uint64_t vm = ballot(valid);
+    return ((uint64_t)(~dh & (uint32_t)(vm >> 32)) << 32) | (~dl & (uint32_t)vm);
 }
 
 // ---- agent-scope synchronisation ----------------------------------------------
