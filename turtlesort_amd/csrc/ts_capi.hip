@@ -149,6 +149,7 @@ struct turtle_sorter {
     // key / index buffers of max_n elements
     void *kin = nullptr, *kout = nullptr, *kt0 = nullptr, *kt1 = nullptr;
     uint32_t *iin = nullptr, *iout = nullptr, *it0 = nullptr, *it1 = nullptr;
+    uint8_t *bins = nullptr;   // per-key bin of the current level (count -> scatter)
     // engine tables
     ts::Ctrl *ctrl = nullptr;
     ts::Ctrl *ctrl_host = nullptr;   // pinned readback copy
@@ -266,6 +267,7 @@ ts::Params make_params(turtle_sorter *h, const void *kin, const uint32_t *iin, v
     p.tile_seg = h->tile_seg;
     p.ldst = h->ldst;
     p.ldsq = h->ldsq;
+    p.bins = h->bins;
     p.levels = h->levels;
     p.segcap = h->segcap;
     p.tilecap = h->tilecap;
@@ -646,7 +648,8 @@ int turtle_create(int dtype, int64_t max_n, int track_index, int compress, turtl
     h->cus = pr.multiProcessorCount;
     const size_t kbytes = (size_t)max_n * kb, ibytes = (size_t)max_n * 4;
     if ((rc = alloc_buf(&h->kin, kbytes, "hipMalloc(keys in)")) || (rc = alloc_buf(&h->kout, kbytes, "hipMalloc(keys out)")) ||
-        (rc = alloc_buf(&h->kt0, kbytes, "hipMalloc(keys tmp0)")) || (rc = alloc_buf(&h->kt1, kbytes, "hipMalloc(keys tmp1)"))) {
+        (rc = alloc_buf(&h->kt0, kbytes, "hipMalloc(keys tmp0)")) || (rc = alloc_buf(&h->kt1, kbytes, "hipMalloc(keys tmp1)")) ||
+        (rc = alloc_buf((void **)&h->bins, (size_t)max_n, "hipMalloc(bins)"))) {
         turtle_destroy(h);
         return rc;
     }
@@ -685,7 +688,7 @@ int turtle_destroy(turtle_handle h)
     if (h->last_stream != h->stream) (void)hipStreamSynchronize(h->last_stream);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     delete h->timer;
-    void *bufs[] = {h->kin, h->kout, h->kt0, h->kt1, h->iin, h->iout, h->it0, h->it1, h->ctrl, h->dev_off};
+    void *bufs[] = {h->kin, h->kout, h->kt0, h->kt1, h->iin, h->iout, h->it0, h->it1, h->bins, h->ctrl, h->dev_off};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     free_tables(h);

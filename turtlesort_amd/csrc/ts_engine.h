@@ -91,6 +91,7 @@ struct Params {
     uint32_t *tile_seg;       // segment id of every tile of the current level
     LdsTask *ldst;
     uint32_t *ldsq;           // LDS tasks ordered by size class
+    uint8_t *bins;            // bin of every key of the current level (count -> scatter)
     Level *levels;
     uint32_t segcap, tilecap, ldscap, pad0;
     uint32_t cutoff;          // LDS sort at or below this length

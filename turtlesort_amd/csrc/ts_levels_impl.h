@@ -17,7 +17,8 @@
 //              S splitters at the sample quantiles (S = 3 is the reference's
 //              3-pivot scheme; 7, 15 and 31 cut a segment into more bins per
 //              pass), reserve the segment's tiles;
-//   k_count    per tile: bin counts (keys only);
+//   k_count    per tile: bin counts; every key's bin is also stored (one
+//              byte) so that the scatter does not search the splitters again;
 //   k_scan1/2  level-wide exclusive scan of the bin-major tile counts: the
 //              destination of every (segment, bin, tile) run;
 //   k_scatter  per tile: stable wave64 ranking into bins, staged in LDS,
@@ -320,9 +321,10 @@ struct LevelK {
                 }
                 load_reg(sh.sp, reg);
                 const U *src = src_keys<U>(p, sg->depth) + sg->begin;
+                uint8_t *bout = p.bins + sg->begin;
                 const uint32_t ts = t * TILE;
                 const uint32_t tl = (sg->len - ts) < TILE ? (sg->len - ts) : TILE;
-                nbytes += (uint64_t)tl * KB;
+                nbytes += (uint64_t)tl * (KB + 1);
                 if (lane() < (uint32_t)NBP) sh.wcnt[w][lane()] = 0;
                 wave_sync();
                 U x[TI];
@@ -340,7 +342,10 @@ struct LevelK {
 #pragma unroll
                 for (int c = 0; c < TI; c++) {
                     const uint32_t e = (uint32_t)w * (TI * WAVE) + c * WAVE + lane();
-                    if (e < tl) atomicAdd(&sh.wcnt[w][bins[c]], 1u);
+                    if (e < tl) {
+                        atomicAdd(&sh.wcnt[w][bins[c]], 1u);
+                        bout[ts + e] = (uint8_t)bins[c];   // the scatter reads the bin back
+                    }
                 }
                 block_sync();
                 if (threadIdx.x < (uint32_t)NB) {
@@ -412,9 +417,8 @@ struct LevelK {
     // (a flag per thread, reported once per tile).
     template <bool FULL>
     static __device__ __forceinline__ void scatter_tile(const Params &p, unsigned char *arena, SharedTile &sh,
-                                                        const Seg *sg, uint32_t t, const Reg &reg)
+                                                        const Seg *sg, uint32_t t)
     {
-        const U flip = (U)p.flip;
         U *stk = (U *)arena;
         uint32_t *sti = (uint32_t *)(arena + TILE * KB);
         unsigned char *stb = arena + TILE * KB + TILE * C::IB;
@@ -432,6 +436,7 @@ struct LevelK {
         const uint32_t tb = sg->tile_base, ntl = sg->ntiles;
         const U *src = src_keys<U>(p, depth) + begin;
         const uint32_t *isrc = src_idx(p, depth) + begin;
+        const uint8_t *bsrc = p.bins + begin;
         U *knext = next_keys<U>(p, depth);
         uint32_t *inext = next_idx(p, depth);
         const uint32_t ts = t * TILE;
@@ -442,27 +447,26 @@ struct LevelK {
         U x[TI];
         uint32_t ix[TI];
         uint32_t pk[TI];   // bin | rank inside the wave's bin << 8
+        // keys and the bins the count pass stored for them
 #pragma unroll
         for (int c = 0; c < TI; c++) {
             const uint32_t e = (uint32_t)w * (TI * WAVE) + c * WAVE + ln;
-            if (FULL) x[c] = src[ts + e];
-            else x[c] = e < tl ? src[ts + e] : (U)0;
+            if (FULL) {
+                x[c] = src[ts + e];
+                pk[c] = bsrc[ts + e];
+            } else {
+                x[c] = e < tl ? src[ts + e] : (U)0;
+                pk[c] = e < tl ? (uint32_t)bsrc[ts + e] : (uint32_t)NB;
+            }
         }
         // rank: running per-wave bin counter (one wave owns wcnt[w], its LDS
         // accesses complete in program order) + rank among the lanes of this
         // step with the same bin
-        // splitter searches in groups of 8 independent chains (a scheduling
-        // barrier between groups bounds the registers they hold)
-#pragma unroll
-        for (int c0 = 0; c0 < TI; c0 += 8) {
-#pragma unroll
-            for (int c = c0; c < c0 + 8 && c < TI; c++) pk[c] = classify<U, S, LOG>((U)(x[c] ^ flip), sh.sp, reg);
-            __builtin_amdgcn_sched_barrier(0);
-        }
 #pragma unroll
         for (int c = 0; c < TI; c++) {
             const uint32_t e = (uint32_t)w * (TI * WAVE) + c * WAVE + ln;
-            const bool valid = FULL || e < tl;
+            const bool valid = (FULL || e < tl) && pk[c] < (uint32_t)NB;
+            bad |= (FULL || e < tl) && pk[c] >= (uint32_t)NB;
             const uint32_t bin = valid ? pk[c] : (uint32_t)NB;
             const uint64_t peers = match_peers<BITS>(bin, valid);
             const uint32_t r = mask_rank(peers);
@@ -540,8 +544,6 @@ struct LevelK {
         const uint32_t ntot = p.ctrl->tile_alloc < p.tilecap ? p.ctrl->tile_alloc : p.tilecap;
         const uint32_t G = p.tiles_per_wg;
         uint64_t nbytes = 0;
-        uint32_t cur = ~0u;
-        Reg reg;
         for (uint32_t c0 = blockIdx.x * G; c0 < ntot; c0 += gridDim.x * G) {
             const uint32_t c1 = c0 + G < ntot ? c0 + G : ntot;
             for (uint32_t g = c0; g < c1; g++) {
@@ -551,17 +553,11 @@ struct LevelK {
                     continue;
                 }
                 const Seg *sg = &p.segs[sid];
-                if (sid != cur) {
-                    split_load<U, S, LOG>(sh.sp, sg);
-                    block_sync();
-                    cur = sid;
-                }
-                load_reg(sh.sp, reg);
                 const uint32_t ts = t * TILE;
                 const uint32_t tl = (sg->len - ts) < TILE ? (sg->len - ts) : TILE;
-                nbytes += 2ull * tl * (KB + C::IB);
-                if (tl == TILE) scatter_tile<true>(p, arena, sh, sg, t, reg);
-                else scatter_tile<false>(p, arena, sh, sg, t, reg);
+                nbytes += (uint64_t)tl * (2 * (KB + C::IB) + 1);
+                if (tl == TILE) scatter_tile<true>(p, arena, sh, sg, t);
+                else scatter_tile<false>(p, arena, sh, sg, t);
             }
         }
         if (threadIdx.x == 0 && nbytes) {
