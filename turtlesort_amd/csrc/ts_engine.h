@@ -20,7 +20,8 @@ enum : uint32_t {
 enum : uint32_t { T_PIVOT = 1, T_COUNT = 2, T_SCATTER = 3, T_LDS = 4 };
 
 // kernel kinds of one engine run (launch timing index; the C ABI's
-// turtle_kernel_kind has the same values)
+// turtle_kernel_kind has the same values; K_LEVEL is no longer launched:
+// the pivot kernel opens each level)
 enum : int {
     K_RESET = 0, K_ROOTS, K_LEVEL, K_PIVOT, K_COUNT, K_SCAN, K_SCATTER, K_CHILDREN, K_LDS, K_LDSPREP, K_KINDS
 };
@@ -47,10 +48,7 @@ struct Ctrl {
     unsigned long long tasks[5];   // indexed by task kind
     // LDS-sort tasks by size class (filled after the last level)
     uint32_t cls_cnt[LDS_CLASSES];      // tasks per class
-    uint32_t cls_off[LDS_CLASSES];      // first slot of the class in the sorted task list
     uint32_t cls_cur[LDS_CLASSES];      // fill cursor per class
-    uint32_t cls_wg[LDS_CLASSES + 1];   // first workgroup step of the class; [LDS_CLASSES] = total
-    uint32_t pad2[3];
 };
 static_assert(sizeof(Ctrl) % 4 == 0, "Ctrl is cleared word by word");
 

@@ -1,6 +1,6 @@
 // ts_levels.hip -- the key-width independent kernels of the sort engine
-// (reset, roots, level opening, LDS-task planning) and the dispatch of
-// engine_levels to the per-width instantiations (ts_levels_kb*.hip).
+// (reset, roots) and the dispatch of engine_levels to the per-width
+// instantiations (ts_levels_kb*.hip).
 #include "ts_levels_impl.h"
 
 namespace ts {
@@ -52,43 +52,6 @@ __global__ void k_roots(Params p, const uint32_t *off, uint32_t nseg, uint32_t s
         t->begin = beg;
         t->len = len;
         t->depth = 0;
-    }
-}
-
-// opens level L: segments [sb, se), no tiles yet
-__global__ void k_level(Params p, uint32_t L)
-{
-    if (threadIdx.x == 0) {
-        const uint32_t sb = L == 0 ? 0u : p.levels[L - 1].se;
-        uint32_t se = p.ctrl->seg_alloc;
-        if (se > p.segcap) se = p.segcap;   // overflow already flagged
-        p.levels[L].sb = sb;
-        p.levels[L].se = se;
-        p.levels[L].tiles = 0;
-        p.ctrl->tile_alloc = 0;
-        if (se > sb && L + 1 > p.ctrl->max_depth) p.ctrl->max_depth = L + 1;
-    }
-}
-
-// LDS-sort plan: first slot and first workgroup step of every size class
-// (class c < crad: cap / 2^c tasks per workgroup step; c >= crad: tasks of
-// c - crad + 1 whole waves, nw / (c - crad + 1) per step)
-__global__ void k_ldsplan(Params p, uint32_t crad, uint32_t nw, uint32_t cap)
-{
-    if (threadIdx.x == 0) {
-        Ctrl *c = p.ctrl;
-        uint32_t off = 0, wg = 0;
-        for (uint32_t k = 0; k < (uint32_t)LDS_CLASSES; k++) {
-            c->cls_off[k] = off;
-            c->cls_wg[k] = wg;
-            c->cls_cur[k] = 0;
-            const uint32_t n = c->cls_cnt[k];
-            off += n;
-            uint32_t P = k < crad ? cap >> k : (k - crad + 1 <= nw ? nw / (k - crad + 1) : 1u);
-            if (P == 0) P = 1;
-            if (n) wg += (n + P - 1) / P;
-        }
-        c->cls_wg[LDS_CLASSES] = wg;
     }
 }
 

@@ -11,9 +11,9 @@
 // the segments of that level, and every segment is cut into tiles that all
 // workgroups share:
 //
-//   k_level    opens level L: its segments are ids [sb, se) of the segment
-//              table (created by the previous level), its tile count is 0;
-//   k_pivot    per segment: sample 1024 keys evenly, sort them in LDS, take
+//   k_pivot    opens level L (its segments are ids [sb, se) of the segment
+//              table, created by the previous level); per segment: sample
+//              1024 keys evenly, sort them in LDS, take
 //              S splitters at the sample quantiles (S = 3 is the reference's
 //              3-pivot scheme; 7, 15 and 31 cut a segment into more bins per
 //              pass), reserve the segment's tiles;
@@ -28,8 +28,7 @@
 //   k_children per segment (one wave): every non-empty non-pivot bin becomes
 //              a segment of level L+1 or, at or below the cutoff, an LDS task.
 // After the last level the LDS tasks are grouped by size class
-// (k_ldscount, k_ldsplan, k_ldsbucket) and k_lds sorts them, 2^CMAX / 2^c
-// tasks of class c per workgroup step.
+// (k_ldscount, k_ldsbucket) and two k_lds kernels sort them.
 //
 // Bins follow the reference's order LL,=pL,L,=p,R,=pR,RR
 // (turtle-sort.cu:755-761) generalised to S splitters: bin 2j+1 holds keys
@@ -176,8 +175,27 @@ __device__ __forceinline__ uint32_t classify(U x, const Split<U, S> &sp, const U
 // ---- set-up kernels ------------------------------------------------------------------
 __global__ void k_reset(Ctrl *c);
 __global__ void k_roots(Params p, const uint32_t *off, uint32_t nseg, uint32_t single_len);
-__global__ void k_level(Params p, uint32_t L);
-__global__ void k_ldsplan(Params p, uint32_t crad, uint32_t nw, uint32_t cap);
+
+// LDS-sort plan from the per-class task counts: first slot of every class
+// in the class-ordered task list and first workgroup step of every class
+// (class c < crad: cap / 2^c tasks per step; c >= crad: tasks of
+// c - crad + 1 whole waves, nw / (c - crad + 1) per step); wg[LDS_CLASSES]
+// is the total.  Every workgroup that needs it computes it (20 entries).
+__device__ __forceinline__ void lds_plan(const Ctrl *c, uint32_t crad, uint32_t nw, uint32_t cap, uint32_t *off,
+                                         uint32_t *wg)
+{
+    uint32_t o = 0, g = 0;
+    for (uint32_t k = 0; k < (uint32_t)LDS_CLASSES; k++) {
+        off[k] = o;
+        wg[k] = g;
+        const uint32_t n = c->cls_cnt[k];
+        o += n;
+        uint32_t P = k < crad ? cap >> k : (k - crad + 1 <= nw ? nw / (k - crad + 1) : 1u);
+        if (P == 0) P = 1;
+        if (n) g += (n + P - 1) / P;
+    }
+    wg[LDS_CLASSES] = g;
+}
 
 template <int KB, bool IDX, int S>
 struct LevelK {
@@ -226,7 +244,18 @@ struct LevelK {
     // independent runs of one block sort.
     static __device__ void pivot(const Params &p, uint32_t L, unsigned char *arena, TaskRec *rec, SharedScan &sh)
     {
-        const Level lv = p.levels[L];
+        // level L = the segments created since level L-1 opened (nothing
+        // appends segments while this kernel runs); workgroup 0 records it
+        // for the later kernels of the level and the host
+        Level lv;
+        lv.sb = L == 0 ? 0u : p.levels[L - 1].se;
+        lv.se = p.ctrl->seg_alloc < p.segcap ? p.ctrl->seg_alloc : p.segcap;   // overflow already flagged
+        lv.tiles = 0;
+        lv.pad = 0;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            p.levels[L] = lv;
+            if (lv.se > lv.sb && L + 1 > p.ctrl->max_depth) p.ctrl->max_depth = L + 1;
+        }
         U *sk = (U *)arena;
         const U flip = (U)p.flip;
         constexpr uint32_t PP = C::PP, SAMPLE = C::SAMPLE;
@@ -571,6 +600,9 @@ struct LevelK {
     static __device__ void children(const Params &p, uint32_t L)
     {
         const Level lv = p.levels[L];
+        // the tiles of this level are done with: the next level's pivot
+        // kernel reserves its tiles from zero
+        if (blockIdx.x == 0 && threadIdx.x == 0) p.ctrl->tile_alloc = 0;
         const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
         const uint32_t nwv = gridDim.x * blockDim.x / WAVE;
         for (uint32_t sid = lv.sb + wid; sid < lv.se; sid += nwv) {
@@ -639,10 +671,12 @@ struct LevelK {
 
     // task ids ordered by class (order inside a class is arbitrary; every
     // task is sorted on its own, so the output does not depend on it)
-    static __device__ void ldsbucket(const Params &p, uint32_t *hist, uint32_t *wbase)
+    static __device__ void ldsbucket(const Params &p, uint32_t *hist, uint32_t *wbase, uint32_t *coff, uint32_t *cwg)
     {
         const uint32_t ntask = p.ctrl->lds_alloc < p.ldscap ? p.ctrl->lds_alloc : p.ldscap;
         const uint32_t stride = gridDim.x * blockDim.x;
+        if (threadIdx.x == 0) lds_plan(p.ctrl, CRAD, NW, CAP, coff, cwg);
+        block_sync();
         for (uint32_t k0 = blockIdx.x * blockDim.x; k0 < ntask; k0 += stride) {
             if (threadIdx.x < (uint32_t)LDS_CLASSES) hist[threadIdx.x] = 0;
             block_sync();
@@ -657,7 +691,7 @@ struct LevelK {
                 wbase[threadIdx.x] = atomic_add(&p.ctrl->cls_cur[threadIdx.x], hist[threadIdx.x]);
             block_sync();
             if (k < ntask) {
-                const uint32_t slot = p.ctrl->cls_off[c] + wbase[c] + lr;
+                const uint32_t slot = coff[c] + wbase[c] + lr;
                 if (slot < p.ldscap) p.ldsq[slot] = k;
                 else range_fail(p, 11);
             }
@@ -696,6 +730,8 @@ struct LevelK {
     static constexpr uint32_t EPT = NW * NBIN / NT;   // scan entries per thread
 
     struct RadixShared {
+        uint32_t coff[LDS_CLASSES];       // LDS-sort plan (lds_plan)
+        uint32_t cwg[LDS_CLASSES + 1];
         uint16_t cnt[NW][NBIN];   // per-wave digit counts, then the digits' first slots
         U wmin[NW], wmax[NW];
         U tmin[NW];
@@ -868,8 +904,10 @@ struct LevelK {
         const U flip = (U)p.flip;
         U *dst = (U *)p.kout;
         const Ctrl *cc = p.ctrl;
+        if (threadIdx.x == 0) lds_plan(cc, CRAD, NW, CAP, rs.coff, rs.cwg);
+        block_sync();
         const uint32_t c0 = RAD ? CRAD : (uint32_t)C::CMIN, clast = RAD ? CLAST : CRAD - 1;
-        const uint32_t vbeg = cc->cls_wg[c0], vend = cc->cls_wg[clast + 1];
+        const uint32_t vbeg = rs.cwg[c0], vend = rs.cwg[clast + 1];
         uint64_t nbytes = 0, ntasks = 0;
         if (threadIdx.x == 0) acc[0] = acc[1] = 0;
         for (uint32_t v = vbeg + blockIdx.x; v < vend; v += gridDim.x) {
@@ -880,17 +918,17 @@ struct LevelK {
             uint32_t tid = threadIdx.x;
             asm volatile("" : "+v"(tid));
             uint32_t c = c0;
-            while (c < clast && v >= cc->cls_wg[c + 1]) c++;
+            while (c < clast && v >= rs.cwg[c + 1]) c++;
             const uint32_t kw = RAD ? c - CRAD + 1 : 0u;
             const uint32_t region = RAD ? kw * WS : 1u << c;     // slots per task
             const uint32_t P = RAD ? (uint32_t)NW / kw : CAP >> c;
-            const uint32_t first = (v - cc->cls_wg[c]) * P;
+            const uint32_t first = (v - rs.cwg[c]) * P;
             const uint32_t cnt = cc->cls_cnt[c];
             const uint32_t nk = first < cnt ? ((cnt - first) < P ? (cnt - first) : P) : 0u;
             for (uint32_t q = threadIdx.x; q < P; q += NT) {
                 TaskRec r{0, 0, 0};
                 if (q < nk) {
-                    const uint32_t slot = cc->cls_off[c] + first + q;
+                    const uint32_t slot = rs.coff[c] + first + q;
                     const uint32_t k = slot < p.ldscap ? p.ldsq[slot] : ~0u;
                     if (k < p.ldscap) {
                         const LdsTask tk = p.ldst[k];
@@ -1050,7 +1088,8 @@ __global__ void __launch_bounds__(256) k_ldsbucket(Params p)
 {
     __shared__ uint32_t hist[LDS_CLASSES];
     __shared__ uint32_t wbase[LDS_CLASSES];
-    LevelK<KB, IDX, S>::ldsbucket(p, hist, wbase);
+    __shared__ uint32_t coff[LDS_CLASSES], cwg[LDS_CLASSES + 1];
+    LevelK<KB, IDX, S>::ldsbucket(p, hist, wbase, coff, cwg);
 }
 
 template <int KB, bool IDX, int S, bool RAD>
@@ -1107,7 +1146,6 @@ hipError_t run_levels(const Params &p, int cus, int wg_per_cu, const uint32_t *d
             TS_LAUNCH(K_ROOTS, k_roots, dim3((nseg + 255) / 256), dim3(256), 0, st, p, dev_off, nseg, single_len);
     }
     for (uint32_t L = L0; L < L0 + nlev && L < (uint32_t)MAX_LEVELS; L++) {
-        TS_LAUNCH(K_LEVEL, k_level, dim3(1), dim3(64), 0, st, p, L);
         TS_LAUNCH(K_PIVOT, (k_pivot<KB, IDX, S>), dim3(g_pivot), dim3(512), 0, st, p, L);
         TS_LAUNCH(K_COUNT, (k_count<KB, IDX, S>), dim3(g_cnt), dim3(512), 0, st, p);
         TS_LAUNCH(K_SCAN, (k_scan1<KB, IDX, S>), dim3(g_small), dim3(512), 0, st, p);
@@ -1117,7 +1155,6 @@ hipError_t run_levels(const Params &p, int cus, int wg_per_cu, const uint32_t *d
     }
     if (last) {
         TS_LAUNCH(K_LDSPREP, (k_ldscount<KB, IDX, S>), dim3(g_small), dim3(256), 0, st, p);
-        TS_LAUNCH(K_LDSPREP, k_ldsplan, dim3(1), dim3(64), 0, st, p, (uint32_t)C::CRAD, (uint32_t)C::NW, C::CAP);
         TS_LAUNCH(K_LDSPREP, (k_ldsbucket<KB, IDX, S>), dim3(g_small), dim3(256), 0, st, p);
         TS_LAUNCH(K_LDS, (k_lds<KB, IDX, S, false>), dim3(g_lds), dim3(512), 0, st, p);
         TS_LAUNCH(K_LDS, (k_lds<KB, IDX, S, true>), dim3(g_lds), dim3(512), 0, st, p);
