@@ -503,10 +503,12 @@ struct LevelK {
             if (valid && r == 0) sh.wcnt[w][bin] = old + (uint32_t)__popcll(peers);
             pk[c] = bin | ((old + r) << 8);
         }
-        // the indices are loaded only now (they are not needed for the
-        // ranking); the barrier and the offset scan cover their latency
+        // the first half of the indices is loaded now (they are not needed
+        // for the ranking; the barrier and the offset scan cover the
+        // latency), the second half while the first is staged
+        constexpr int H = TI / 2;
 #pragma unroll
-        for (int c = 0; c < TI; c++) {
+        for (int c = 0; c < H; c++) {
             const uint32_t e = (uint32_t)w * (TI * WAVE) + c * WAVE + ln;
             if (IDX) ix[c] = (FULL || e < tl) ? isrc[ts + e] : 0u;
         }
@@ -533,18 +535,29 @@ struct LevelK {
         }
         block_sync();
 #pragma unroll
-        for (int c = 0; c < TI; c++) {
-            const uint32_t e = (uint32_t)w * (TI * WAVE) + c * WAVE + ln;
-            const uint32_t bin = pk[c] & 0xffu;
-            uint32_t pos = sh.woff[w][bin] + (pk[c] >> 8);
-            const bool valid = FULL || e < tl;
-            bad |= valid && pos >= tl;
-            pos &= TILE - 1;                   // stays inside the staging arrays
-            if (valid) {
-                stk[pos] = x[c];
-                if (IDX) sti[pos] = ix[c];
-                stb[pos] = (unsigned char)bin;
+        for (int h = 0; h < 2; h++) {
+            if (h == 1) {
+#pragma unroll
+                for (int c = H; c < TI; c++) {
+                    const uint32_t e = (uint32_t)w * (TI * WAVE) + c * WAVE + ln;
+                    if (IDX) ix[c] = (FULL || e < tl) ? isrc[ts + e] : 0u;
+                }
             }
+#pragma unroll
+            for (int c = h * H; c < (h + 1) * H; c++) {
+                const uint32_t e = (uint32_t)w * (TI * WAVE) + c * WAVE + ln;
+                const uint32_t bin = pk[c] & 0xffu;
+                uint32_t pos = sh.woff[w][bin] + (pk[c] >> 8);
+                const bool valid = FULL || e < tl;
+                bad |= valid && pos >= tl;
+                pos &= TILE - 1;                   // stays inside the staging arrays
+                if (valid) {
+                    stk[pos] = x[c];
+                    if (IDX) sti[pos] = ix[c];
+                    stb[pos] = (unsigned char)bin;
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
         }
         block_sync();
 #pragma nounroll
