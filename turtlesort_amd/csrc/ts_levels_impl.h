@@ -149,7 +149,8 @@ __device__ __forceinline__ void split_load(Split<U, S> &sp, const Seg *sg)
 // bin of order value x: lt = #splitters < x; equal to a splitter -> 2*lt+1
 // (lt then indexes the lowest equal splitter), else 2*lt
 template <typename U, int S, int LOG>
-__device__ __forceinline__ uint32_t classify(U x, const Split<U, S> &sp, const U (&reg)[S <= 7 ? S : 1])
+__device__ __forceinline__ uint32_t classify(U x, const Split<U, S> &sp, const U (&reg)[S <= 7 ? S : 1],
+                                             const U (&reg3)[3])
 {
     uint32_t lt;
     bool eq;
@@ -163,9 +164,14 @@ __device__ __forceinline__ uint32_t classify(U x, const Split<U, S> &sp, const U
         }
         eq = le > lt;
     } else {
-        uint32_t i = 1;
+        // the top two levels of the tree come from registers (reg3: nodes
+        // 1, 2, 3, the same for every lane), the rest from LDS
+        const U r1 = reg3[0], r2 = reg3[1], r3 = reg3[2];
+        const uint32_t c1 = r1 < x ? 1u : 0u;
+        const U t = c1 ? r3 : r2;
+        uint32_t i = 4 + 2 * c1 + (t < x ? 1u : 0u);
 #pragma unroll
-        for (int l = 0; l < LOG; l++) i = 2 * i + ((sp.tree[i] < x) ? 1u : 0u);
+        for (int l = 2; l < LOG; l++) i = 2 * i + ((sp.tree[i] < x) ? 1u : 0u);
         lt = i - (S + 1);
         eq = lt < (uint32_t)S && sp.srt[lt] == x;
     }
@@ -220,11 +226,14 @@ struct LevelK {
         uint32_t begin, len, depth;
     };
 
-    static __device__ __forceinline__ void load_reg(const Split<U, S> &sp, Reg &reg)
+    static __device__ __forceinline__ void load_reg(const Split<U, S> &sp, Reg &reg, U (&reg3)[3])
     {
         if constexpr (S <= 7) {
 #pragma unroll
             for (int j = 0; j < S; j++) reg[j] = sp.srt[j];
+        } else {
+#pragma unroll
+            for (int j = 0; j < 3; j++) reg3[j] = sp.tree[j + 1];
         }
     }
 
@@ -334,6 +343,7 @@ struct LevelK {
         uint64_t nbytes = 0;
         uint32_t cur = ~0u;
         Reg reg;
+        U reg3[3];
         for (uint32_t c0 = blockIdx.x * G; c0 < ntot; c0 += gridDim.x * G) {
             const uint32_t c1 = c0 + G < ntot ? c0 + G : ntot;
             for (uint32_t g = c0; g < c1; g++) {
@@ -348,7 +358,7 @@ struct LevelK {
                     block_sync();
                     cur = sid;
                 }
-                load_reg(sh.sp, reg);
+                load_reg(sh.sp, reg, reg3);
                 const U *src = src_keys<U>(p, sg->depth) + sg->begin;
                 uint8_t *bout = p.bins + sg->begin;
                 const uint32_t ts = t * TILE;
@@ -367,7 +377,7 @@ struct LevelK {
                 // lanes of equal bins are serialised by the LDS
                 uint32_t bins[TI];
 #pragma unroll
-                for (int c = 0; c < TI; c++) bins[c] = classify<U, S, LOG>((U)(x[c] ^ flip), sh.sp, reg);
+                for (int c = 0; c < TI; c++) bins[c] = classify<U, S, LOG>((U)(x[c] ^ flip), sh.sp, reg, reg3);
 #pragma unroll
                 for (int c = 0; c < TI; c++) {
                     const uint32_t e = (uint32_t)w * (TI * WAVE) + c * WAVE + lane();
