@@ -13,10 +13,10 @@ value      = keys sorted per second over the K timed steps (Gkeys/s), wall
 roofline   = the dominant kernel (largest summed device time over the timed
              steps, from the per-launch HIP events recorded on the stream the
              kernels run on): its algorithmic bytes (device counters: count
-             pass K B/key, partition scatter and LDS sort 2(K+I) B/key) over
-             its summed duration, against the 8 TB/s HBM3E peak.  "traffic"
-             is the HBM bytes per launch measured with rocprofv3 counters,
-             read from the committed profile summary when present.
+             pass K+1 B/key, partition scatter 2(K+I)+1 B/key, LDS sort
+             2(K+I) B/key) over its summed duration, against the 8 TB/s HBM3E
+             peak.  "traffic" is the HBM bytes per launch measured with
+             rocprofv3 counters, read from the committed profile summary.
 end_to_end = the same algorithmic bytes of all passes over the whole sort's
              device time (the "fraction of HBM roofline across the actual
              number of passes")
