@@ -61,7 +61,6 @@ struct Cfg {
     static constexpr uint32_t TILE = NT * TI;
     static constexpr uint32_t CAP = NT * SI;
     static constexpr uint32_t SAMPLE = 1024;
-    static constexpr uint32_t PP = CAP / SAMPLE;          // segments per pivot workgroup step
     static constexpr uint32_t IB = IDX ? 4 : 0;
     // LDS-sort size classes (see LevelK::lds_class): merge classes
     // [CMIN, CRAD) of n_eff = 2^c keys, radix classes [CRAD, CLAST] of
@@ -246,91 +245,6 @@ struct LevelK {
         if (g < sg->tile_base) return false;
         t = g - sg->tile_base;
         return t < sg->ntiles && span_ok(p, sg->begin, sg->len) && (uint64_t)t * TILE < sg->len;
-    }
-
-    // ---------------------------------------------------------------- pivot
-    // PP segments per workgroup step: their 1024-key samples are sorted as
-    // independent runs of one block sort.
-    static __device__ void pivot(const Params &p, uint32_t L, unsigned char *arena, TaskRec *rec, SharedScan &sh)
-    {
-        // level L = the segments created since level L-1 opened (nothing
-        // appends segments while this kernel runs); workgroup 0 records it
-        // for the later kernels of the level and the host
-        Level lv;
-        lv.sb = L == 0 ? 0u : p.levels[L - 1].se;
-        lv.se = p.ctrl->seg_alloc < p.segcap ? p.ctrl->seg_alloc : p.segcap;   // overflow already flagged
-        lv.tiles = 0;
-        lv.pad = 0;
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            p.levels[L] = lv;
-            if (lv.se > lv.sb && L + 1 > p.ctrl->max_depth) p.ctrl->max_depth = L + 1;
-        }
-        U *sk = (U *)arena;
-        const U flip = (U)p.flip;
-        constexpr uint32_t PP = C::PP, SAMPLE = C::SAMPLE;
-        for (uint32_t s0 = lv.sb + blockIdx.x * PP; s0 < lv.se; s0 += gridDim.x * PP) {
-            const uint32_t nh = (lv.se - s0) < PP ? (lv.se - s0) : PP;
-            if (threadIdx.x < nh) {
-                const Seg *sg = &p.segs[s0 + threadIdx.x];
-                TaskRec r{sg->begin, sg->len, sg->depth};
-                if (!span_ok(p, r.begin, r.len)) {
-                    range_fail(p, 3);
-                    r.len = 0;
-                }
-                rec[threadIdx.x] = r;
-            }
-            block_sync();
-            {
-                U kv[SI];
-#pragma unroll
-                for (int r = 0; r < SI; r++) {
-                    const uint32_t i = threadIdx.x + (uint32_t)r * NT;
-                    const uint32_t q = i / SAMPLE, j = i % SAMPLE;
-                    kv[r] = key_max<U>();
-                    if (q < nh) {
-                        const TaskRec tr = rec[q];
-                        const uint32_t m = tr.len < SAMPLE ? tr.len : SAMPLE;
-                        if (j < m) {
-                            const uint64_t pos = ((2ull * j + 1ull) * (uint64_t)tr.len) / (2ull * m);
-                            kv[r] = (U)(src_keys<U>(p, tr.depth)[tr.begin + (uint32_t)pos] ^ flip);
-                        }
-                    }
-                }
-                U *const mk = sk + pidx<SI>(threadIdx.x);   // pidx(tid + r*NT) = pidx(tid) + r*(NT + NT/SI)
-#pragma unroll
-                for (int r = 0; r < SI; r++) mk[r * (NT + NT / SI)] = kv[r];
-            }
-            block_sync();
-            block_sort<U, false, NT, SI>(sk, nullptr, CAP, SAMPLE);
-            if (threadIdx.x < nh) {
-                const uint32_t q = threadIdx.x;
-                const TaskRec r = rec[q];
-                Seg *sg = &p.segs[s0 + q];
-                uint32_t tb = ~0u, ntiles = 0;
-                if (r.len > 0) {
-                    const uint32_t m = r.len < SAMPLE ? r.len : SAMPLE;
-                    for (int j = 0; j < S; j++)
-                        sg->split[j] = (uint64_t)sk[pidx<SI>(q * SAMPLE + ((uint32_t)(j + 1) * m) / (S + 1))];
-                    ntiles = (r.len + TILE - 1) / TILE;
-                    tb = atomic_add(&p.ctrl->tile_alloc, ntiles);
-                    if (tb > p.tilecap || ntiles > p.tilecap - tb) {
-                        atomic_or(&p.ctrl->error, ERR_TILES);
-                        tb = ~0u;
-                    }
-                }
-                sg->ntiles = tb == ~0u ? 0u : ntiles;
-                sg->tile_base = tb == ~0u ? 0u : tb;
-                rec[q].len = tb == ~0u ? 0u : ntiles;   // reused: tiles to map
-                rec[q].begin = tb;
-            }
-            if (threadIdx.x == 0) atomic_add64((unsigned long long *)&p.ctrl->tasks[T_PIVOT], (unsigned long long)nh);
-            block_sync();
-            for (uint32_t q = 0; q < nh; q++) {
-                const uint32_t tb = rec[q].begin, nt = rec[q].len;
-                for (uint32_t t = threadIdx.x; t < nt; t += NT) p.tile_seg[tb + t] = s0 + q;
-            }
-            block_sync();
-        }
     }
 
     // ---------------------------------------------------------------- count
@@ -762,6 +676,11 @@ struct LevelK {
         uint32_t maxbits;
     };
 
+    struct PivotShared {
+        uint16_t cnt[NW][NBIN];       // per-wave digit counts, then the digits' first positions
+        U buf[NW][C::SAMPLE];         // per-wave sample
+    };
+
     static __device__ __forceinline__ U wave_min(U x)
     {
 #pragma unroll
@@ -944,7 +863,7 @@ struct LevelK {
             while (c < clast && v >= rs.cwg[c + 1]) c++;
             const uint32_t kw = RAD ? c - CRAD + 1 : 0u;
             const uint32_t region = RAD ? kw * WS : 1u << c;     // slots per task
-            const uint32_t P = RAD ? (uint32_t)NW / kw : CAP >> c;
+            const uint32_t P = RAD ? (uint32_t)NW / (kw ? kw : 1u) : CAP >> c;
             const uint32_t first = (v - rs.cwg[c]) * P;
             const uint32_t cnt = cc->cls_cnt[c];
             const uint32_t nk = first < cnt ? ((cnt - first) < P ? (cnt - first) : P) : 0u;
@@ -1047,6 +966,137 @@ struct LevelK {
             atomic_add64(&p.ctrl->tasks[T_LDS], acc[1]);
         }
     }
+
+    // ---------------------------------------------------------------- pivot
+    // One wave per segment (8 per workgroup step): the wave samples 1024 keys
+    // at evenly spaced positions into registers (16 per lane), sorts them by
+    // stable LSD radix passes of RW-bit digits over the sample's key range
+    // (ranks from wave64 match masks and per-wave digit counters, no block
+    // barrier), takes S splitters at the sample quantiles and reserves the
+    // segment's tiles.
+    static constexpr uint32_t SPL = C::SAMPLE / WAVE;   // sample keys per lane
+
+    static __device__ void pivot(const Params &p, uint32_t L, PivotShared &ps)
+    {
+        // level L = the segments created since level L-1 opened (nothing
+        // appends segments while this kernel runs); workgroup 0 records it
+        // for the later kernels of the level and the host
+        Level lv;
+        lv.sb = L == 0 ? 0u : p.levels[L - 1].se;
+        lv.se = p.ctrl->seg_alloc < p.segcap ? p.ctrl->seg_alloc : p.segcap;   // overflow already flagged
+        lv.tiles = 0;
+        lv.pad = 0;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            p.levels[L] = lv;
+            if (lv.se > lv.sb && L + 1 > p.ctrl->max_depth) p.ctrl->max_depth = L + 1;
+        }
+        const U flip = (U)p.flip;
+        constexpr uint32_t SAMPLE = C::SAMPLE;
+        const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE), l = lane();
+        U *const b = ps.buf[w];
+        uint16_t *const cn = ps.cnt[w];
+        uint32_t done = 0;
+        for (uint32_t s0 = lv.sb + blockIdx.x * NW; s0 < lv.se; s0 += gridDim.x * NW) {
+            const uint32_t sid = s0 + w;
+            if (sid >= lv.se) continue;   // wave-uniform
+            done++;
+            Seg *sg = &p.segs[sid];
+            const uint32_t begin = sg->begin, len = sg->len, depth = sg->depth;
+            if (!span_ok(p, begin, len)) {
+                if (l == 0) {
+                    range_fail(p, 3);
+                    sg->ntiles = 0;
+                    sg->tile_base = 0;
+                }
+                continue;
+            }
+            const uint32_t m = len < SAMPLE ? len : SAMPLE;
+            const U *src = src_keys<U>(p, depth) + begin;
+            U k[SPL];
+            U mn = key_max<U>(), mx = 0;
+#pragma unroll
+            for (int i = 0; i < (int)SPL; i++) {
+                const uint32_t j = (uint32_t)i * WAVE + l;
+                k[i] = key_max<U>();
+                if (j < m) {
+                    // (2j+1) len / 2m; m is SAMPLE (a power of two) whenever len >= SAMPLE
+                    const uint64_t num = (2ull * j + 1ull) * (uint64_t)len;
+                    const uint64_t pos = m == SAMPLE ? num >> (ilog2(SAMPLE) + 1) : num / (2ull * m);
+                    k[i] = (U)(src[(uint32_t)pos] ^ flip);
+                    mn = k[i] < mn ? k[i] : mn;
+                    mx = k[i] > mx ? k[i] : mx;
+                }
+            }
+            mn = wave_min(mn);
+            mx = wave_max(mx);
+            const uint64_t span = mx > mn ? (uint64_t)(U)(mx - mn) : 0ull;
+            const uint32_t bits = span ? (uint32_t)(64 - __clzll(span)) : 0u;
+            const uint32_t npass = (bits + RW - 1) / RW;
+            for (uint32_t pass = 0; pass < npass; pass++) {
+                const uint32_t sh = pass * RW;
+                for (uint32_t d = l; d < NBIN; d += WAVE) cn[d] = 0;
+                wave_sync();
+                uint32_t dr[SPL];
+#pragma unroll
+                for (int i = 0; i < (int)SPL; i++) {
+                    // after a pass the sample's keys are at its first m positions
+                    const bool real = (uint32_t)i * WAVE + l < m;
+                    const uint32_t d = real ? (uint32_t)((U)(k[i] - mn) >> sh) & (NBIN - 1) : NBIN - 1;
+                    const uint64_t peers = match_peers<RW>(d, true);
+                    const uint32_t r = mask_rank(peers);
+                    const uint32_t old = cn[d];
+                    if (r == 0) cn[d] = (uint16_t)(old + (uint32_t)__popcll(peers));
+                    dr[i] = d | ((old + r) << 16);
+                }
+                wave_sync();
+                // exclusive scan of the NBIN digit counts: lane l owns NBIN/WAVE digits
+                {
+                    constexpr uint32_t DPL = NBIN / WAVE;
+                    uint32_t c8[DPL], loc = 0;
+#pragma unroll
+                    for (uint32_t q = 0; q < DPL; q++) {
+                        c8[q] = cn[l * DPL + q];
+                        loc += c8[q];
+                    }
+                    uint32_t run = wave_incl_scan(loc) - loc;
+#pragma unroll
+                    for (uint32_t q = 0; q < DPL; q++) {
+                        cn[l * DPL + q] = (uint16_t)run;
+                        run += c8[q];
+                    }
+                }
+                wave_sync();
+#pragma unroll
+                for (int i = 0; i < (int)SPL; i++) b[(cn[dr[i] & 0xffffu] + (dr[i] >> 16)) & (SAMPLE - 1)] = k[i];
+                wave_sync();
+#pragma unroll
+                for (int i = 0; i < (int)SPL; i++) k[i] = b[(uint32_t)i * WAVE + l];
+                wave_sync();
+            }
+            if (npass == 0) {
+#pragma unroll
+                for (int i = 0; i < (int)SPL; i++) b[(uint32_t)i * WAVE + l] = k[i];
+                wave_sync();
+            }
+            if (l < (uint32_t)S) sg->split[l] = (uint64_t)b[((l + 1) * m) / (S + 1)];
+            const uint32_t ntiles = (len + TILE - 1) / TILE;
+            uint32_t tb = 0;
+            if (l == 0) {
+                tb = atomic_add(&p.ctrl->tile_alloc, ntiles);
+                if (tb > p.tilecap || ntiles > p.tilecap - tb) {
+                    atomic_or(&p.ctrl->error, ERR_TILES);
+                    tb = ~0u;
+                }
+                sg->ntiles = tb == ~0u ? 0u : ntiles;
+                sg->tile_base = tb == ~0u ? 0u : tb;
+            }
+            tb = __shfl(tb, 0, WAVE);
+            if (tb != ~0u)
+                for (uint32_t t = l; t < ntiles; t += WAVE) p.tile_seg[tb + t] = sid;
+            wave_sync();
+        }
+        if (l == 0 && done) atomic_add64((unsigned long long *)&p.ctrl->tasks[T_PIVOT], (unsigned long long)done);
+    }
 };
 
 // ---- kernels ----------------------------------------------------------------------------
@@ -1054,10 +1104,8 @@ template <int KB, bool IDX, int S>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k_pivot(Params p, uint32_t L)
 {
     typedef LevelK<KB, IDX, S> K;
-    __shared__ __align__(16) unsigned char arena[Cfg<KB, IDX, S>::SORT_KEY_BYTES];
-    __shared__ typename K::TaskRec rec[Cfg<KB, IDX, S>::PP];
-    __shared__ typename K::SharedScan sh;
-    K::pivot(p, L, arena, rec, sh);
+    __shared__ typename K::PivotShared ps;
+    K::pivot(p, L, ps);
 }
 
 template <int KB, bool IDX, int S>
@@ -1140,22 +1188,20 @@ hipError_t run_levels(const Params &p, int cus, int wg_per_cu, const uint32_t *d
                       uint32_t single_len, uint32_t L0, uint32_t nlev, bool first, bool last, hipStream_t st,
                       LaunchTimer *tm)
 {
-    typedef Cfg<KB, IDX, S> C;
     // workgroups per CU of each kernel (computed once, thread-safe)
     struct Occ {
-        int tile, cnt, pivot, lds;
+        int tile, cnt, lds;
     };
     static const Occ oc = [] {
         Occ o;
         o.cnt = occ<KB, IDX, S>((const void *)k_count<KB, IDX, S>, 512);
-        o.pivot = occ<KB, IDX, S>((const void *)k_pivot<KB, IDX, S>, 512);
         o.lds = occ<KB, IDX, S>((const void *)k_lds<KB, IDX, S, true>, 512);
         o.tile = occ<KB, IDX, S>((const void *)k_scatter<KB, IDX, S>, 512);
         return o;
     }();
-    const int o_tile = oc.tile, o_cnt = oc.cnt, o_pivot = oc.pivot, o_lds = oc.lds;
+    const int o_tile = oc.tile, o_cnt = oc.cnt, o_lds = oc.lds;
     auto lim = [&](int o) { return cus * ((wg_per_cu > 0 && wg_per_cu < o) ? wg_per_cu : o); };
-    const int g_tile = lim(o_tile), g_cnt = lim(o_cnt), g_pivot = cus * o_pivot, g_lds = cus * o_lds;
+    const int g_tile = lim(o_tile), g_cnt = lim(o_cnt), g_pivot = cus, g_lds = cus * o_lds;
     const int g_small = cus * 2;
 #define TS_LAUNCH(kind, ...)                  \
     do {                                      \
