@@ -1,8 +1,7 @@
 // The reference's MultiSort test driver (main.cpp:4-55) against the drop-in
-// header: 32,000 arrays x 2038 keys, BlockSize 32, 10 repetitions.  The
-// reference uses float keys; this build sorts integer keys, so the keys are
-// int with the same rand()-based fill.  Each array is checked for equality
-// with std::sort of its input, not only for sortedness.
+// header: 32,000 arrays x 2038 float keys, BlockSize 32, 10 repetitions, the
+// same rand() * 3.14f fill.  Each array is checked for equality with
+// std::sort of its input, not only for sortedness.
 #include <algorithm>
 #include <cstdlib>
 #include <iostream>
@@ -11,7 +10,7 @@
 
 int main(int argc, char **argv)
 {
-    using Type = int;
+    using Type = float;
     constexpr int arrSize = 2038;
     constexpr int blockSize = 32;
     int numArraysToSort = 1000 * blockSize;
@@ -22,7 +21,7 @@ int main(int argc, char **argv)
     std::vector<Type> hostData(n), expect(n);
     srand(12345);
     for (int k = 0; k < reps; k++) {
-        for (int i = 0; i < n; i++) hostData[i] = rand() * 3;
+        for (int i = 0; i < n; i++) hostData[i] = rand() * 3.14f;
         expect = hostData;
         for (int a = 0; a < numArraysToSort; a++)
             std::sort(expect.begin() + (size_t)a * arrSize, expect.begin() + (size_t)(a + 1) * arrSize);

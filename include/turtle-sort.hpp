@@ -91,13 +91,18 @@ namespace Turtle {
 // README.md:146 uses Turtle::Bench
 using Bench = TurtleGlobals::Bench;
 
+// key type code of T: the reference's ten instantiations (turtle-sort.cu:956-1174)
+// plus 64-bit integers
 template <typename T>
 constexpr int dtypeOf()
 {
-    static_assert(std::is_integral<T>::value && !std::is_same<T, bool>::value,
-                  "TurtleSort keys are integer types on this build");
+    static_assert((std::is_integral<T>::value && !std::is_same<T, bool>::value) || std::is_same<T, float>::value ||
+                      std::is_same<T, double>::value,
+                  "TurtleSort keys are integer, float or double types");
     static_assert(sizeof(T) == 1 || sizeof(T) == 2 || sizeof(T) == 4 || sizeof(T) == 8, "unsupported key width");
-    return sizeof(T) == 1   ? (std::is_signed<T>::value ? TURTLE_INT8 : TURTLE_UINT8)
+    return std::is_same<T, float>::value    ? TURTLE_FLOAT32
+           : std::is_same<T, double>::value ? TURTLE_FLOAT64
+           : sizeof(T) == 1 ? (std::is_signed<T>::value ? TURTLE_INT8 : TURTLE_UINT8)
            : sizeof(T) == 2 ? (std::is_signed<T>::value ? TURTLE_INT16 : TURTLE_UINT16)
            : sizeof(T) == 4 ? (std::is_signed<T>::value ? TURTLE_INT32 : TURTLE_UINT32)
                             : (std::is_signed<T>::value ? TURTLE_INT64 : TURTLE_UINT64);

@@ -37,12 +37,17 @@ typedef enum turtle_status {
 
 /* Key types.  The reference instantiates its kernels for char, short, int,
  * long and their unsigned forms plus float/double (turtle-sort.cu:956-1174);
- * long is 32-bit on its only platform (MSVC), so 64-bit keys are added here. */
+ * long is 32-bit on its only platform (MSVC), so 64-bit keys are added here.
+ * Floating-point keys sort in IEEE 754 totalOrder: -NaN < -inf < ... < -0 <
+ * +0 < ... < +inf < +NaN.  That agrees with < on every pair < orders, so on
+ * NaN-free input the keys equal std::sort's; the two zeros compare equal
+ * under <, and here -0 precedes +0 (the stable index follows that order). */
 typedef enum turtle_dtype {
     TURTLE_INT8 = 1, TURTLE_UINT8 = 2,
     TURTLE_INT16 = 3, TURTLE_UINT16 = 4,
     TURTLE_INT32 = 5, TURTLE_UINT32 = 6,
-    TURTLE_INT64 = 7, TURTLE_UINT64 = 8
+    TURTLE_INT64 = 7, TURTLE_UINT64 = 8,
+    TURTLE_FLOAT32 = 9, TURTLE_FLOAT64 = 10
 } turtle_dtype;
 
 typedef struct turtle_sorter *turtle_handle;

@@ -13,7 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "build", "liboracle.so")
 
 DTYPES = {np.dtype(np.int8): 1, np.dtype(np.uint8): 2, np.dtype(np.int16): 3, np.dtype(np.uint16): 4,
-          np.dtype(np.int32): 5, np.dtype(np.uint32): 6, np.dtype(np.int64): 7, np.dtype(np.uint64): 8}
+          np.dtype(np.int32): 5, np.dtype(np.uint32): 6, np.dtype(np.int64): 7, np.dtype(np.uint64): 8,
+          np.dtype(np.float32): 9, np.dtype(np.float64): 10}
 
 _lib = None
 
@@ -55,6 +56,15 @@ def load():
 
 def _p(a):
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def order(keys):
+    """order values (uint64) of the oracle's key map: their unsigned order is
+    the sort order of the keys"""
+    k = np.ascontiguousarray(keys)
+    m = np.empty(k.size, dtype=np.uint64)
+    assert load().ts_oracle_to_ord(DTYPES[k.dtype], _p(k), _p(m), k.size) == 0
+    return m
 
 
 def turtle_sort(keys, idx=None):

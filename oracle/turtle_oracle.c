@@ -24,6 +24,16 @@
  * same answer on the mapped value, so the restated algorithm takes the same
  * decisions and produces the same key sequence.
  *
+ * float and double keys (instantiated by the reference, turtle-sort.cu:
+ * 1136-1174, and sorted by its only test, main.cpp:6-30) map through their
+ * bit pattern: non-negative values get the sign bit set, negative values
+ * every bit inverted, then widen unchanged.  That is IEEE 754 totalOrder
+ * (-NaN < -inf < ... < -0 < +0 < ... < +inf < +NaN).  On NaN-free input
+ * it answers every comparison < answers, except that -0 < +0 where < sees
+ * them equal; the reference's own comparisons (turtle-sort.cu:562-567) are
+ * not a strict weak order once NaN is present, so NaN placement is a policy
+ * of this build, not a restatement.
+ *
  * Two deliberate differences from the reference's observable behaviour, both
  * defects documented against the reference source:
  *   - the merge moves the tracked index (the reference moves the key into the
@@ -46,6 +56,11 @@
  *                              conservation
  *   ts_oracle_multisort     -> libc qsort per array
  *   ts_oracle_splitmix64    -> published splitmix64 output for seed 1234567
+ *   float/double order map  -> numpy's sort (bit-exact on NaN-free input
+ *                              without mixed zeros); with NaN and both zeros:
+ *                              value equality with numpy's sort of the
+ *                              non-NaN part, -0 before +0, sign-set NaNs
+ *                              first and the others last
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -56,7 +71,8 @@
  * (a test checks that they agree) */
 enum {
     ORC_INT8 = 1, ORC_UINT8 = 2, ORC_INT16 = 3, ORC_UINT16 = 4,
-    ORC_INT32 = 5, ORC_UINT32 = 6, ORC_INT64 = 7, ORC_UINT64 = 8
+    ORC_INT32 = 5, ORC_UINT32 = 6, ORC_INT64 = 7, ORC_UINT64 = 8,
+    ORC_FLOAT32 = 9, ORC_FLOAT64 = 10
 };
 
 int ts_oracle_dtype_size(int dtype)
@@ -64,8 +80,8 @@ int ts_oracle_dtype_size(int dtype)
     switch (dtype) {
     case ORC_INT8: case ORC_UINT8: return 1;
     case ORC_INT16: case ORC_UINT16: return 2;
-    case ORC_INT32: case ORC_UINT32: return 4;
-    case ORC_INT64: case ORC_UINT64: return 8;
+    case ORC_INT32: case ORC_UINT32: case ORC_FLOAT32: return 4;
+    case ORC_INT64: case ORC_UINT64: case ORC_FLOAT64: return 8;
     default: return 0;
     }
 }
@@ -74,6 +90,13 @@ int ts_oracle_dtype_size(int dtype)
 /* order-preserving map                                                      */
 /* ------------------------------------------------------------------------ */
 #define SIGN64 0x8000000000000000ULL
+#define SIGN32 0x80000000u
+
+/* IEEE bit pattern -> totalOrder rank of the same width, and back */
+static uint32_t f32_ord(uint32_t b) { return (b & SIGN32) ? ~b : (b | SIGN32); }
+static uint32_t f32_unord(uint32_t o) { return (o & SIGN32) ? (o & ~SIGN32) : ~o; }
+static uint64_t f64_ord(uint64_t b) { return (b & SIGN64) ? ~b : (b | SIGN64); }
+static uint64_t f64_unord(uint64_t o) { return (o & SIGN64) ? (o & ~SIGN64) : ~o; }
 
 int ts_oracle_to_ord(int dtype, const void *in, uint64_t *out, int64_t n)
 {
@@ -87,6 +110,8 @@ int ts_oracle_to_ord(int dtype, const void *in, uint64_t *out, int64_t n)
     case ORC_UINT32: for (i = 0; i < n; i++) out[i] = ((const uint32_t *)in)[i]; break;
     case ORC_INT64:  for (i = 0; i < n; i++) out[i] = (uint64_t)((const int64_t *)in)[i] ^ SIGN64; break;
     case ORC_UINT64: for (i = 0; i < n; i++) out[i] = ((const uint64_t *)in)[i]; break;
+    case ORC_FLOAT32: for (i = 0; i < n; i++) out[i] = f32_ord(((const uint32_t *)in)[i]); break;
+    case ORC_FLOAT64: for (i = 0; i < n; i++) out[i] = f64_ord(((const uint64_t *)in)[i]); break;
     default: return -1;
     }
     return 0;
@@ -104,6 +129,8 @@ int ts_oracle_from_ord(int dtype, const uint64_t *in, void *out, int64_t n)
     case ORC_UINT32: for (i = 0; i < n; i++) ((uint32_t *)out)[i] = (uint32_t)in[i]; break;
     case ORC_INT64:  for (i = 0; i < n; i++) ((int64_t *)out)[i] = (int64_t)(in[i] ^ SIGN64); break;
     case ORC_UINT64: for (i = 0; i < n; i++) ((uint64_t *)out)[i] = in[i]; break;
+    case ORC_FLOAT32: for (i = 0; i < n; i++) ((uint32_t *)out)[i] = f32_unord((uint32_t)in[i]); break;
+    case ORC_FLOAT64: for (i = 0; i < n; i++) ((uint64_t *)out)[i] = f64_unord(in[i]); break;
     default: return -1;
     }
     return 0;

@@ -34,8 +34,8 @@ def test_status_and_dtype_helpers():
     lib = _lib.load()
     assert lib.turtle_status_string(0) == b"ok"
     assert lib.turtle_status_string(7) == b"work queue overflow"
-    assert [lib.turtle_dtype_size(d) for d in range(1, 9)] == [1, 1, 2, 2, 4, 4, 8, 8]
-    assert lib.turtle_dtype_size(0) == 0 and lib.turtle_dtype_size(9) == 0
+    assert [lib.turtle_dtype_size(d) for d in range(1, 11)] == [1, 1, 2, 2, 4, 4, 8, 8, 4, 8]
+    assert lib.turtle_dtype_size(0) == 0 and lib.turtle_dtype_size(11) == 0
 
 
 def test_dtype_codes_match_oracle(orc):

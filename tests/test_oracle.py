@@ -328,3 +328,72 @@ def test_cpu_baselines_sort(orc):
         np.testing.assert_array_equal(k[oi], ok)
     sec, ok, oi = orc.cpu_sort(3, k, i, want_output=True)
     np.testing.assert_array_equal(oi, np.argsort(k, kind="stable"))
+
+
+# ---------------------------------------------------------------- float keys
+def float_specials(dt):
+    f = np.finfo(dt)
+    return np.array([np.inf, -np.inf, f.max, -f.max, f.tiny, -f.tiny, f.smallest_subnormal,
+                     -f.smallest_subnormal, 1.0, -1.0, 0.5, -0.5], dtype=dt)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_float_order_map_vs_numpy(orc, dtype):
+    """NaN-free keys with one sign of zero: the order map sorts exactly as
+    numpy does, and maps back to the same bits"""
+    dt = np.dtype(dtype)
+    ui = np.dtype(f"u{dt.itemsize}")
+    bits = wl.gen_keys(dtype, 50000, 77, 0)
+    x = np.concatenate([bits[~np.isnan(bits)], float_specials(dt), np.zeros(5, dtype=dt)])
+    x = x[~((x == 0) & np.signbit(x))]                      # +0 only
+    np.random.default_rng(5).shuffle(x)
+    m = orc.order(x)
+    np.testing.assert_array_equal(np.argsort(m, kind="stable"), np.argsort(x, kind="stable"))
+    back = np.empty_like(x)
+    orc.load().ts_oracle_from_ord(orc.DTYPES[dt], p(m), p(back), x.size)
+    np.testing.assert_array_equal(back.view(ui), x.view(ui))
+    ok, _ = orc.turtle_sort(x)
+    np.testing.assert_array_equal(ok.view(ui), np.sort(x).view(ui))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_float_sort_nan_and_zero_policy(orc, dtype):
+    """NaNs of both signs and both zeros: the non-NaN keys equal numpy's sort
+    by value, -0 precedes +0, sign-set NaNs come first and the rest last,
+    and the key bits are a permutation of the input's"""
+    dt = np.dtype(dtype)
+    ui = np.dtype(f"u{dt.itemsize}")
+    rng = np.random.default_rng(11)
+    vals = np.concatenate([float_specials(dt), np.array([0.0, -0.0, np.nan, -np.nan], dtype=dt),
+                           rng.standard_normal(20).astype(dt)])
+    x = vals[rng.integers(0, vals.size, 20000)]
+    # NaN payloads of both signs
+    xb = x.view(ui).copy()
+    nanb = np.isnan(x)
+    xb[nanb] |= rng.integers(1, 1 << 20, nanb.sum()).astype(ui)
+    x = xb.view(dt)
+    ok, oi = orc.turtle_sort(x, np.arange(x.size, dtype=np.int32))
+    nneg = int((np.isnan(x) & np.signbit(x)).sum())
+    npos = int((np.isnan(x) & ~np.signbit(x)).sum())
+    assert np.isnan(ok[:nneg]).all() and np.signbit(ok[:nneg]).all()
+    assert np.isnan(ok[ok.size - npos:]).all() and not np.signbit(ok[ok.size - npos:]).any()
+    mid = ok[nneg:ok.size - npos]
+    np.testing.assert_array_equal(mid, np.sort(x[~np.isnan(x)]))          # by value
+    z = mid[mid == 0]
+    assert z.size > 0 and np.signbit(z).any() and not np.signbit(z).all()
+    nz = int(np.signbit(z).sum())
+    assert np.signbit(z[:nz]).all() and not np.signbit(z[nz:]).any()      # -0 before +0
+    np.testing.assert_array_equal(np.sort(ok.view(ui)), np.sort(x.view(ui)))
+    np.testing.assert_array_equal(x[oi].view(ui), ok.view(ui))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_float_order_map_brute_small(orc, dtype):
+    """every ordered pair of a set of specials compares under the map as the
+    IEEE comparison does wherever that is ordered (brute force)"""
+    dt = np.dtype(dtype)
+    x = np.concatenate([float_specials(dt), np.array([0.0, 3.0, -7.25, 1e-30, -1e-30], dtype=dt)])
+    m = orc.order(x)
+    for i in range(x.size):
+        for j in range(x.size):
+            assert (m[i] < m[j]) == (x[i] < x[j])

@@ -24,7 +24,7 @@ ERR_STATE = 9
 ERR_BUDGET = 10
 
 # dtype codes (turtle_dtype)
-INT8, UINT8, INT16, UINT16, INT32, UINT32, INT64, UINT64 = range(1, 9)
+INT8, UINT8, INT16, UINT16, INT32, UINT32, INT64, UINT64, FLOAT32, FLOAT64 = range(1, 11)
 
 
 class TurtleError(RuntimeError):

@@ -23,6 +23,7 @@ _NP_TO_CODE = {
     np.dtype(np.int16): _lib.INT16, np.dtype(np.uint16): _lib.UINT16,
     np.dtype(np.int32): _lib.INT32, np.dtype(np.uint32): _lib.UINT32,
     np.dtype(np.int64): _lib.INT64, np.dtype(np.uint64): _lib.UINT64,
+    np.dtype(np.float32): _lib.FLOAT32, np.dtype(np.float64): _lib.FLOAT64,
 }
 CODE_TO_NP = {v: k for k, v in _NP_TO_CODE.items()}
 NAME_TO_CODE = {str(k): v for k, v in _NP_TO_CODE.items()}
@@ -38,7 +39,8 @@ def dtype_code(dtype):
         import torch
         if isinstance(dtype, torch.dtype):
             m = {torch.int8: _lib.INT8, torch.uint8: _lib.UINT8, torch.int16: _lib.INT16,
-                 torch.int32: _lib.INT32, torch.int64: _lib.INT64}
+                 torch.int32: _lib.INT32, torch.int64: _lib.INT64,
+                 torch.float32: _lib.FLOAT32, torch.float64: _lib.FLOAT64}
             for name in ("uint16", "uint32", "uint64"):
                 td = getattr(torch, name, None)
                 if td is not None:

@@ -54,6 +54,8 @@ int dtype_bytes(int dtype)
     case TURTLE_INT16: case TURTLE_UINT16: return 2;
     case TURTLE_INT32: case TURTLE_UINT32: return 4;
     case TURTLE_INT64: case TURTLE_UINT64: return 8;
+    case TURTLE_FLOAT32: return 4;
+    case TURTLE_FLOAT64: return 8;
     default: return 0;
     }
 }
@@ -63,10 +65,16 @@ bool dtype_signed(int dtype)
     return dtype == TURTLE_INT8 || dtype == TURTLE_INT16 || dtype == TURTLE_INT32 || dtype == TURTLE_INT64;
 }
 
+bool dtype_float(int dtype) { return dtype == TURTLE_FLOAT32 || dtype == TURTLE_FLOAT64; }
+
+// sign-bit mask of the order transform of the device key traits
 uint64_t flip_of(int dtype)
 {
-    return dtype_signed(dtype) ? (1ull << (8 * dtype_bytes(dtype) - 1)) : 0ull;
+    return dtype_signed(dtype) || dtype_float(dtype) ? (1ull << (8 * dtype_bytes(dtype) - 1)) : 0ull;
 }
+
+// key type code of the kernel templates
+int kt_of(int dtype) { return dtype_bytes(dtype) | (dtype_float(dtype) ? ts::KT_FLOAT : 0); }
 
 double now_s()
 {
@@ -138,7 +146,7 @@ struct EventTimer : ts::LaunchTimer {
 }  // namespace
 
 struct turtle_sorter {
-    int dtype = 0, kb = 0;
+    int dtype = 0, kb = 0, kt = 0;   // key type, key bytes, kernel key code
     uint64_t flip = 0;
     int track = 0;
     int64_t max_n = 0;
@@ -353,12 +361,12 @@ int run_engine(turtle_sorter *h, const void *kin, const uint32_t *iin, void *kou
     if (capturing) {
         nlev += 2;   // no read-back inside a graph: two spare levels
         if (nlev > (uint32_t)ts::MAX_LEVELS) nlev = ts::MAX_LEVELS;
-        HIPCHK(ts::engine_levels(p, h->kb, idx, S, h->cus, wpc, dev_off, nseg, (uint32_t)total, 0, nlev, true, true, st),
+        HIPCHK(ts::engine_levels(p, h->kt, idx, S, h->cus, wpc, dev_off, nseg, (uint32_t)total, 0, nlev, true, true, st),
                "engine launch");
         h->levels_run = nlev;
     } else if (nlev == 0) {
         // every root fits the LDS sort: no partition level, no read-back
-        HIPCHK(ts::engine_levels(p, h->kb, idx, S, h->cus, wpc, dev_off, nseg, (uint32_t)total, 0, 0, true, true, st,
+        HIPCHK(ts::engine_levels(p, h->kt, idx, S, h->cus, wpc, dev_off, nseg, (uint32_t)total, 0, 0, true, true, st,
                                  h->timer),
                "engine launch");
         h->levels_run = 0;
@@ -367,7 +375,7 @@ int run_engine(turtle_sorter *h, const void *kin, const uint32_t *iin, void *kou
         bool first = true;
         for (;;) {
             if (L0 + nlev > (uint32_t)ts::MAX_LEVELS) nlev = ts::MAX_LEVELS - L0;
-            HIPCHK(ts::engine_levels(p, h->kb, idx, S, h->cus, wpc, dev_off, nseg, (uint32_t)total, L0, nlev, first,
+            HIPCHK(ts::engine_levels(p, h->kt, idx, S, h->cus, wpc, dev_off, nseg, (uint32_t)total, L0, nlev, first,
                                      false, st, h->timer),
                    "engine launch");
             first = false;
@@ -383,7 +391,7 @@ int run_engine(turtle_sorter *h, const void *kin, const uint32_t *iin, void *kou
             nlev = 4;
         }
         h->levels_run = L0;
-        HIPCHK(ts::engine_levels(p, h->kb, idx, S, h->cus, wpc, dev_off, nseg, (uint32_t)total, L0, 0, false, true, st,
+        HIPCHK(ts::engine_levels(p, h->kt, idx, S, h->cus, wpc, dev_off, nseg, (uint32_t)total, L0, 0, false, true, st,
                                  h->timer),
                "engine launch");
     }
@@ -633,6 +641,7 @@ int turtle_create(int dtype, int64_t max_n, int track_index, int compress, turtl
     turtle_sorter *h = new turtle_sorter();
     h->dtype = dtype;
     h->kb = kb;
+    h->kt = kt_of(dtype);
     h->flip = flip_of(dtype);
     h->track = track_index ? 1 : 0;
     h->max_n = max_n;
@@ -890,7 +899,7 @@ int turtle_multisort_device(turtle_handle h, int dtype, int64_t arr_size, const 
     hipStream_t st = stream_of(h, stream);
     if ((uint64_t)arr_size <= ts::multisort_max_size(h->kb)) {
         HIPCHK(hipEventRecord(h->ev0, st), "hipEventRecord");
-        HIPCHK(ts::multisort_run(h->kb, h->flip, (uint32_t)arr_size, keys_in, keys_out, (uint64_t)num_arrays, st),
+        HIPCHK(ts::multisort_run(h->kt, h->flip, (uint32_t)arr_size, keys_in, keys_out, (uint64_t)num_arrays, st),
                "multisort launch");
         HIPCHK(hipEventRecord(h->ev1, st), "hipEventRecord");
         h->timed = true;

@@ -11,13 +11,47 @@ constexpr int WAVE = 64;
 
 // ---- key traits ------------------------------------------------------------
 // Keys are moved in their stored representation.  Comparisons use an
-// unsigned "order" value: signed keys have their sign bit flipped, which maps
-// the signed order onto the unsigned order of the same width.
+// unsigned "order" value of the same width:
+//  - unsigned integers: the key itself (flip = 0);
+//  - signed integers: the sign bit flipped (flip = sign mask), which maps the
+//    signed order onto the unsigned order;
+//  - IEEE binary32/binary64 (FL): non-negative values get the sign bit set,
+//    negative ones every bit inverted (flip = sign mask).  The result orders
+//    as IEEE 754 totalOrder: -NaN < -inf < ... < -0 < +0 < ... < +inf < +NaN,
+//    which agrees with the < of the type on every pair < orders.
 template <int KB> struct KeyT;
-template <> struct KeyT<1> { typedef uint8_t U; };
-template <> struct KeyT<2> { typedef uint16_t U; };
-template <> struct KeyT<4> { typedef uint32_t U; };
-template <> struct KeyT<8> { typedef uint64_t U; };
+template <> struct KeyT<1> { typedef uint8_t U; typedef int8_t I; };
+template <> struct KeyT<2> { typedef uint16_t U; typedef int16_t I; };
+template <> struct KeyT<4> { typedef uint32_t U; typedef int32_t I; };
+template <> struct KeyT<8> { typedef uint64_t U; typedef int64_t I; };
+
+// all-ones when the top bit of x is set, else 0
+template <typename U>
+__device__ __forceinline__ U top_mask(U x)
+{
+    typedef typename KeyT<sizeof(U)>::I I;
+    return (U)((I)x >> (8 * sizeof(U) - 1));
+}
+
+// stored representation -> order value
+template <typename U, bool FL>
+__device__ __forceinline__ U to_ord(U x, U flip)
+{
+    if constexpr (FL)
+        return (U)(x ^ (flip | top_mask(x)));
+    else
+        return (U)(x ^ flip);
+}
+
+// order value -> stored representation
+template <typename U, bool FL>
+__device__ __forceinline__ U from_ord(U x, U flip)
+{
+    if constexpr (FL)
+        return (U)(x ^ (flip | (U)~top_mask(x)));
+    else
+        return (U)(x ^ flip);
+}
 
 template <typename U>
 __device__ __forceinline__ U key_max() { return (U)~(U)0; }

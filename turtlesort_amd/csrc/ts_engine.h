@@ -26,6 +26,12 @@ enum : int {
     K_RESET = 0, K_ROOTS, K_LEVEL, K_PIVOT, K_COUNT, K_SCAN, K_SCATTER, K_CHILDREN, K_LDS, K_LDSPREP, K_KINDS
 };
 
+// Key type code of the engine and MultiSort templates: the key width in
+// bytes (1, 2, 4, 8), plus KT_FLOAT for IEEE floating-point keys.
+constexpr int KT_FLOAT = 16;
+constexpr int kt_bytes(int kt) { return kt & 15; }
+constexpr bool kt_float(int kt) { return (kt & KT_FLOAT) != 0; }
+
 constexpr int MAX_SPLIT = 31;
 constexpr int MAX_BINS = 2 * MAX_SPLIT + 1;
 constexpr int MAX_LEVELS = 64;
@@ -96,7 +102,7 @@ struct Params {
     uint32_t tiles_per_wg;    // tiles a workgroup takes per step of the tile loops
     uint32_t n_total;         // length of every key / index buffer
     uint32_t pad1;
-    uint64_t flip;            // sign-bit mask for signed keys, else 0
+    uint64_t flip;            // sign-bit mask for signed and floating-point keys, else 0
 };
 
 }  // namespace ts

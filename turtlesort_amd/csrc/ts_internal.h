@@ -20,15 +20,17 @@ bool engine_supports(int splitters);   // splitter counts compiled in
 uint32_t engine_tile(int kb);          // elements per partition tile
 uint32_t engine_cap(int kb);           // largest segment the LDS sort takes
 // Enqueues on st: (first) reset + roots, then levels [L0, L0+nlev), then
-// (last) the LDS sorts.  dev_off: nseg+1 segment offsets (uint32) in device
-// memory, or nullptr for the single segment [0, single_len).
-hipError_t engine_levels(const Params &p, int kb, bool idx, int splitters, int cus, int wg_per_cu,
+// (last) the LDS sorts.  kt: key type code (KT_FLOAT | key bytes).  dev_off:
+// nseg+1 segment offsets (uint32) in device memory, or nullptr for the single
+// segment [0, single_len).
+hipError_t engine_levels(const Params &p, int kt, bool idx, int splitters, int cus, int wg_per_cu,
                          const uint32_t *dev_off, uint32_t nseg, uint32_t single_len, uint32_t L0, uint32_t nlev,
                          bool first, bool last, hipStream_t st, LaunchTimer *timer = nullptr);
 
 // MultiSort kernels (ts_multisort.hip): arrays of arr_size keys, in -> out.
-// Returns hipErrorInvalidValue when arr_size exceeds multisort_max_size(kb).
-hipError_t multisort_run(int kb, uint64_t flip, uint32_t arr_size, const void *in, void *out,
+// kt: key type code.  Returns hipErrorInvalidValue when arr_size exceeds
+// multisort_max_size(kt).
+hipError_t multisort_run(int kt, uint64_t flip, uint32_t arr_size, const void *in, void *out,
                          uint64_t num_arrays, hipStream_t st);
 uint32_t multisort_max_size(int kb);
 

@@ -1,6 +1,6 @@
 // ts_levels.hip -- the key-width independent kernels of the sort engine
-// (reset, roots) and the dispatch of engine_levels to the per-width
-// instantiations (ts_levels_kb*.hip).
+// (reset, roots) and the dispatch of engine_levels to the per-key-type
+// instantiations (ts_levels_kb*.hip, ts_levels_kf*.hip).
 #include "ts_levels_impl.h"
 
 namespace ts {
@@ -59,16 +59,18 @@ bool engine_supports(int splitters) { return splitters == 3 || splitters == 7 ||
 uint32_t engine_tile(int kb) { return kb >= 8 ? 512u * 8u : 512u * 16u; }
 uint32_t engine_cap(int kb) { return kb >= 8 ? 512u * 8u : 512u * 16u; }
 
-hipError_t engine_levels(const Params &p, int kb, bool idx, int splitters, int cus, int wg_per_cu,
+hipError_t engine_levels(const Params &p, int kt, bool idx, int splitters, int cus, int wg_per_cu,
                          const uint32_t *dev_off, uint32_t nseg, uint32_t single_len, uint32_t L0, uint32_t nlev,
                          bool first, bool last, hipStream_t st, LaunchTimer *timer)
 {
     levels_fn f = nullptr;
-    switch (kb) {
+    switch (kt) {
     case 1: f = pick_kb1(idx, splitters); break;
     case 2: f = pick_kb2(idx, splitters); break;
     case 4: f = pick_kb4(idx, splitters); break;
     case 8: f = pick_kb8(idx, splitters); break;
+    case 4 | KT_FLOAT: f = pick_kf4(idx, splitters); break;
+    case 8 | KT_FLOAT: f = pick_kf8(idx, splitters); break;
     default: break;
     }
     if (!f) return hipErrorInvalidValue;

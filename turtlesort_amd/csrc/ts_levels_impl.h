@@ -47,8 +47,10 @@
 
 namespace ts {
 
-template <int KB, bool IDX, int S>
+template <int KT, bool IDX, int S>
 struct Cfg {
+    static constexpr int KB = kt_bytes(KT);               // key bytes
+    static constexpr bool FL = kt_float(KT);              // IEEE floating-point keys
     typedef typename KeyT<KB>::U U;
     static constexpr int NT = 512;
     static constexpr int NW = NT / WAVE;
@@ -202,10 +204,12 @@ __device__ __forceinline__ void lds_plan(const Ctrl *c, uint32_t crad, uint32_t 
     wg[LDS_CLASSES] = g;
 }
 
-template <int KB, bool IDX, int S>
+template <int KT, bool IDX, int S>
 struct LevelK {
-    typedef Cfg<KB, IDX, S> C;
+    typedef Cfg<KT, IDX, S> C;
     typedef typename C::U U;
+    static constexpr int KB = C::KB;
+    static constexpr bool FL = C::FL;
     static constexpr int NT = C::NT, NW = C::NW, NB = C::NB, NBP = C::NBP, BITS = C::BITS, LOG = C::LOG;
     static constexpr int TI = C::TI, SI = C::SI;
     static constexpr uint32_t TILE = C::TILE, CAP = C::CAP;
@@ -291,7 +295,7 @@ struct LevelK {
                 // lanes of equal bins are serialised by the LDS
                 uint32_t bins[TI];
 #pragma unroll
-                for (int c = 0; c < TI; c++) bins[c] = classify<U, S, LOG>((U)(x[c] ^ flip), sh.sp, reg, reg3);
+                for (int c = 0; c < TI; c++) bins[c] = classify<U, S, LOG>(to_ord<U, FL>(x[c], flip), sh.sp, reg, reg3);
 #pragma unroll
                 for (int c = 0; c < TI; c++) {
                     const uint32_t e = (uint32_t)w * (TI * WAVE) + c * WAVE + lane();
@@ -731,7 +735,7 @@ struct LevelK {
                 k[i] = key_max<U>();
                 v[i] = 0u;
                 if (j < tr.len) {
-                    k[i] = (U)(src[j] ^ flip);
+                    k[i] = to_ord<U, FL>(src[j], flip);
                     if (IDX) v[i] = isrc[j];
                     mn = k[i] < mn ? k[i] : mn;
                     mx = k[i] > mx ? k[i] : mx;
@@ -900,7 +904,7 @@ struct LevelK {
                     if (t < nk) {
                         const TaskRec tr = rec[t];
                         if (j < tr.len) {
-                            dst[tr.begin + j] = (U)(sk[s] ^ flip);
+                            dst[tr.begin + j] = from_ord<U, FL>(sk[s], flip);
                             if (IDX) p.iout[tr.begin + j] = si[s];
                         }
                     }
@@ -921,7 +925,7 @@ struct LevelK {
                         kv[r] = key_max<U>();
                         iv[r] = 0u;
                         if (j < tr.len) {
-                            kv[r] = (U)(src_keys<U>(p, tr.depth)[tr.begin + j] ^ flip);
+                            kv[r] = to_ord<U, FL>(src_keys<U>(p, tr.depth)[tr.begin + j], flip);
                             if (IDX) iv[r] = src_idx(p, tr.depth)[tr.begin + j];
                         }
                     }
@@ -949,7 +953,7 @@ struct LevelK {
                     const uint32_t q = i >> c, j = i & (n_eff - 1);
                     const TaskRec tr = rec[q];
                     if (j < tr.len) {
-                        dst[tr.begin + j] = (U)(mk[r * (NT + NT / SI)] ^ flip);
+                        dst[tr.begin + j] = from_ord<U, FL>(mk[r * (NT + NT / SI)], flip);
                         if (IDX) p.iout[tr.begin + j] = mi[r * (NT + NT / SI)];
                     }
                 }
@@ -1022,7 +1026,7 @@ struct LevelK {
                     // (2j+1) len / 2m; m is SAMPLE (a power of two) whenever len >= SAMPLE
                     const uint64_t num = (2ull * j + 1ull) * (uint64_t)len;
                     const uint64_t pos = m == SAMPLE ? num >> (ilog2(SAMPLE) + 1) : num / (2ull * m);
-                    k[i] = (U)(src[(uint32_t)pos] ^ flip);
+                    k[i] = to_ord<U, FL>(src[(uint32_t)pos], flip);
                     mn = k[i] < mn ? k[i] : mn;
                     mx = k[i] > mx ? k[i] : mx;
                 }
@@ -1100,82 +1104,82 @@ struct LevelK {
 };
 
 // ---- kernels ----------------------------------------------------------------------------
-template <int KB, bool IDX, int S>
+template <int KT, bool IDX, int S>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k_pivot(Params p, uint32_t L)
 {
-    typedef LevelK<KB, IDX, S> K;
+    typedef LevelK<KT, IDX, S> K;
     __shared__ typename K::PivotShared ps;
     K::pivot(p, L, ps);
 }
 
-template <int KB, bool IDX, int S>
+template <int KT, bool IDX, int S>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k_count(Params p)
 {
-    typedef LevelK<KB, IDX, S> K;
+    typedef LevelK<KT, IDX, S> K;
     __shared__ typename K::SharedTile sh;
     K::count(p, sh);
 }
 
-template <int KB, bool IDX, int S>
+template <int KT, bool IDX, int S>
 __global__ void __launch_bounds__(512) k_scan1(Params p)
 {
-    typedef LevelK<KB, IDX, S> K;
+    typedef LevelK<KT, IDX, S> K;
     __shared__ typename K::SharedScan sh;
     K::scan1(p, sh);
 }
 
-template <int KB, bool IDX, int S>
+template <int KT, bool IDX, int S>
 __global__ void __launch_bounds__(512) k_scan2(Params p)
 {
-    typedef LevelK<KB, IDX, S> K;
+    typedef LevelK<KT, IDX, S> K;
     __shared__ typename K::SharedScan sh;
     K::scan2(p, sh);
 }
 
-template <int KB, bool IDX, int S>
+template <int KT, bool IDX, int S>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k_scatter(Params p)
 {
-    typedef LevelK<KB, IDX, S> K;
-    __shared__ __align__(16) unsigned char arena[Cfg<KB, IDX, S>::SCAT_BYTES];
+    typedef LevelK<KT, IDX, S> K;
+    __shared__ __align__(16) unsigned char arena[Cfg<KT, IDX, S>::SCAT_BYTES];
     __shared__ typename K::SharedTile sh;
     K::scatter(p, arena, sh);
 }
 
-template <int KB, bool IDX, int S>
+template <int KT, bool IDX, int S>
 __global__ void __launch_bounds__(256) k_children(Params p, uint32_t L)
 {
-    LevelK<KB, IDX, S>::children(p, L);
+    LevelK<KT, IDX, S>::children(p, L);
 }
 
-template <int KB, bool IDX, int S>
+template <int KT, bool IDX, int S>
 __global__ void __launch_bounds__(256) k_ldscount(Params p)
 {
     __shared__ uint32_t hist[LDS_CLASSES];
-    LevelK<KB, IDX, S>::ldscount(p, hist);
+    LevelK<KT, IDX, S>::ldscount(p, hist);
 }
 
-template <int KB, bool IDX, int S>
+template <int KT, bool IDX, int S>
 __global__ void __launch_bounds__(256) k_ldsbucket(Params p)
 {
     __shared__ uint32_t hist[LDS_CLASSES];
     __shared__ uint32_t wbase[LDS_CLASSES];
     __shared__ uint32_t coff[LDS_CLASSES], cwg[LDS_CLASSES + 1];
-    LevelK<KB, IDX, S>::ldsbucket(p, hist, wbase, coff, cwg);
+    LevelK<KT, IDX, S>::ldsbucket(p, hist, wbase, coff, cwg);
 }
 
-template <int KB, bool IDX, int S, bool RAD>
+template <int KT, bool IDX, int S, bool RAD>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k_lds(Params p)
 {
-    typedef LevelK<KB, IDX, S> K;
-    __shared__ __align__(16) unsigned char arena[Cfg<KB, IDX, S>::SORT_BYTES];
-    __shared__ typename K::TaskRec rec[Cfg<KB, IDX, S>::CAP >> Cfg<KB, IDX, S>::CMIN];
+    typedef LevelK<KT, IDX, S> K;
+    __shared__ __align__(16) unsigned char arena[Cfg<KT, IDX, S>::SORT_BYTES];
+    __shared__ typename K::TaskRec rec[Cfg<KT, IDX, S>::CAP >> Cfg<KT, IDX, S>::CMIN];
     __shared__ typename K::RadixShared rs;
     __shared__ unsigned long long acc[2];
     K::template lds<RAD>(p, arena, rec, rs, acc);
 }
 
 // ---- host side ------------------------------------------------------------------------
-template <int KB, bool IDX, int S>
+template <int KT, bool IDX, int S>
 static int occ(const void *k, int threads)
 {
     int nb = 0;
@@ -1183,7 +1187,7 @@ static int occ(const void *k, int threads)
     return nb;
 }
 
-template <int KB, bool IDX, int S>
+template <int KT, bool IDX, int S>
 hipError_t run_levels(const Params &p, int cus, int wg_per_cu, const uint32_t *dev_off, uint32_t nseg,
                       uint32_t single_len, uint32_t L0, uint32_t nlev, bool first, bool last, hipStream_t st,
                       LaunchTimer *tm)
@@ -1194,9 +1198,9 @@ hipError_t run_levels(const Params &p, int cus, int wg_per_cu, const uint32_t *d
     };
     static const Occ oc = [] {
         Occ o;
-        o.cnt = occ<KB, IDX, S>((const void *)k_count<KB, IDX, S>, 512);
-        o.lds = occ<KB, IDX, S>((const void *)k_lds<KB, IDX, S, true>, 512);
-        o.tile = occ<KB, IDX, S>((const void *)k_scatter<KB, IDX, S>, 512);
+        o.cnt = occ<KT, IDX, S>((const void *)k_count<KT, IDX, S>, 512);
+        o.lds = occ<KT, IDX, S>((const void *)k_lds<KT, IDX, S, true>, 512);
+        o.tile = occ<KT, IDX, S>((const void *)k_scatter<KT, IDX, S>, 512);
         return o;
     }();
     const int o_tile = oc.tile, o_cnt = oc.cnt, o_lds = oc.lds;
@@ -1215,18 +1219,18 @@ hipError_t run_levels(const Params &p, int cus, int wg_per_cu, const uint32_t *d
             TS_LAUNCH(K_ROOTS, k_roots, dim3((nseg + 255) / 256), dim3(256), 0, st, p, dev_off, nseg, single_len);
     }
     for (uint32_t L = L0; L < L0 + nlev && L < (uint32_t)MAX_LEVELS; L++) {
-        TS_LAUNCH(K_PIVOT, (k_pivot<KB, IDX, S>), dim3(g_pivot), dim3(512), 0, st, p, L);
-        TS_LAUNCH(K_COUNT, (k_count<KB, IDX, S>), dim3(g_cnt), dim3(512), 0, st, p);
-        TS_LAUNCH(K_SCAN, (k_scan1<KB, IDX, S>), dim3(g_small), dim3(512), 0, st, p);
-        TS_LAUNCH(K_SCAN, (k_scan2<KB, IDX, S>), dim3(g_small), dim3(512), 0, st, p);
-        TS_LAUNCH(K_SCATTER, (k_scatter<KB, IDX, S>), dim3(g_tile), dim3(512), 0, st, p);
-        TS_LAUNCH(K_CHILDREN, (k_children<KB, IDX, S>), dim3(g_small), dim3(256), 0, st, p, L);
+        TS_LAUNCH(K_PIVOT, (k_pivot<KT, IDX, S>), dim3(g_pivot), dim3(512), 0, st, p, L);
+        TS_LAUNCH(K_COUNT, (k_count<KT, IDX, S>), dim3(g_cnt), dim3(512), 0, st, p);
+        TS_LAUNCH(K_SCAN, (k_scan1<KT, IDX, S>), dim3(g_small), dim3(512), 0, st, p);
+        TS_LAUNCH(K_SCAN, (k_scan2<KT, IDX, S>), dim3(g_small), dim3(512), 0, st, p);
+        TS_LAUNCH(K_SCATTER, (k_scatter<KT, IDX, S>), dim3(g_tile), dim3(512), 0, st, p);
+        TS_LAUNCH(K_CHILDREN, (k_children<KT, IDX, S>), dim3(g_small), dim3(256), 0, st, p, L);
     }
     if (last) {
-        TS_LAUNCH(K_LDSPREP, (k_ldscount<KB, IDX, S>), dim3(g_small), dim3(256), 0, st, p);
-        TS_LAUNCH(K_LDSPREP, (k_ldsbucket<KB, IDX, S>), dim3(g_small), dim3(256), 0, st, p);
-        TS_LAUNCH(K_LDS, (k_lds<KB, IDX, S, false>), dim3(g_lds), dim3(512), 0, st, p);
-        TS_LAUNCH(K_LDS, (k_lds<KB, IDX, S, true>), dim3(g_lds), dim3(512), 0, st, p);
+        TS_LAUNCH(K_LDSPREP, (k_ldscount<KT, IDX, S>), dim3(g_small), dim3(256), 0, st, p);
+        TS_LAUNCH(K_LDSPREP, (k_ldsbucket<KT, IDX, S>), dim3(g_small), dim3(256), 0, st, p);
+        TS_LAUNCH(K_LDS, (k_lds<KT, IDX, S, false>), dim3(g_lds), dim3(512), 0, st, p);
+        TS_LAUNCH(K_LDS, (k_lds<KT, IDX, S, true>), dim3(g_lds), dim3(512), 0, st, p);
     }
 #undef TS_LAUNCH
     return hipGetLastError();
@@ -1235,14 +1239,14 @@ hipError_t run_levels(const Params &p, int cus, int wg_per_cu, const uint32_t *d
 typedef hipError_t (*levels_fn)(const Params &, int, int, const uint32_t *, uint32_t, uint32_t, uint32_t, uint32_t,
                                 bool, bool, hipStream_t, LaunchTimer *);
 
-template <int KB>
+template <int KT>
 levels_fn pick_kb(bool idx, int S)
 {
     switch (S) {
-    case 3: return idx ? run_levels<KB, true, 3> : run_levels<KB, false, 3>;
-    case 7: return idx ? run_levels<KB, true, 7> : run_levels<KB, false, 7>;
-    case 15: return idx ? run_levels<KB, true, 15> : run_levels<KB, false, 15>;
-    case 31: return idx ? run_levels<KB, true, 31> : run_levels<KB, false, 31>;
+    case 3: return idx ? run_levels<KT, true, 3> : run_levels<KT, false, 3>;
+    case 7: return idx ? run_levels<KT, true, 7> : run_levels<KT, false, 7>;
+    case 15: return idx ? run_levels<KT, true, 15> : run_levels<KT, false, 15>;
+    case 31: return idx ? run_levels<KT, true, 31> : run_levels<KT, false, 31>;
     default: return nullptr;
     }
 }
@@ -1251,5 +1255,7 @@ levels_fn pick_kb1(bool idx, int S);
 levels_fn pick_kb2(bool idx, int S);
 levels_fn pick_kb4(bool idx, int S);
 levels_fn pick_kb8(bool idx, int S);
+levels_fn pick_kf4(bool idx, int S);
+levels_fn pick_kf8(bool idx, int S);
 
 }  // namespace ts

@@ -14,7 +14,7 @@
 
 namespace ts {
 
-template <typename U, int NT, int ITEMS>
+template <typename U, bool FL, int NT, int ITEMS>
 __global__ void __launch_bounds__(NT) ms_kernel(const U *__restrict__ in, U *__restrict__ out, uint32_t arr_size,
                                                  U flip)
 {
@@ -24,13 +24,13 @@ __global__ void __launch_bounds__(NT) ms_kernel(const U *__restrict__ in, U *__r
     uint32_t n_eff = next_pow2_dev(arr_size);
     if (n_eff < (uint32_t)ITEMS) n_eff = ITEMS;
     for (uint32_t i = threadIdx.x; i < n_eff; i += NT)
-        sk[pidx<ITEMS>(i)] = i < arr_size ? (U)(in[base + i] ^ flip) : key_max<U>();
+        sk[pidx<ITEMS>(i)] = i < arr_size ? to_ord<U, FL>(in[base + i], flip) : key_max<U>();
     block_sync();
     block_sort<U, false, NT, ITEMS>(sk, nullptr, n_eff, n_eff);
-    for (uint32_t i = threadIdx.x; i < arr_size; i += NT) out[base + i] = (U)(sk[pidx<ITEMS>(i)] ^ flip);
+    for (uint32_t i = threadIdx.x; i < arr_size; i += NT) out[base + i] = from_ord<U, FL>(sk[pidx<ITEMS>(i)], flip);
 }
 
-template <typename U>
+template <typename U, bool FL>
 static hipError_t ms_launch(uint64_t flip, uint32_t arr_size, const void *in, void *out, uint64_t num_arrays,
                             hipStream_t st)
 {
@@ -45,13 +45,13 @@ static hipError_t ms_launch(uint64_t flip, uint32_t arr_size, const void *in, vo
         const U *ii = i + a0 * arr_size;
         U *oo = o + a0 * arr_size;
         if (ne <= 256)
-            hipLaunchKernelGGL((ms_kernel<U, 64, 4>), dim3((uint32_t)na), dim3(64), 0, st, ii, oo, arr_size, f);
+            hipLaunchKernelGGL((ms_kernel<U, FL, 64, 4>), dim3((uint32_t)na), dim3(64), 0, st, ii, oo, arr_size, f);
         else if (ne <= 1024)
-            hipLaunchKernelGGL((ms_kernel<U, 128, 8>), dim3((uint32_t)na), dim3(128), 0, st, ii, oo, arr_size, f);
+            hipLaunchKernelGGL((ms_kernel<U, FL, 128, 8>), dim3((uint32_t)na), dim3(128), 0, st, ii, oo, arr_size, f);
         else if (ne <= 4096)
-            hipLaunchKernelGGL((ms_kernel<U, 256, 16>), dim3((uint32_t)na), dim3(256), 0, st, ii, oo, arr_size, f);
+            hipLaunchKernelGGL((ms_kernel<U, FL, 256, 16>), dim3((uint32_t)na), dim3(256), 0, st, ii, oo, arr_size, f);
         else if (ne <= 8192)
-            hipLaunchKernelGGL((ms_kernel<U, 512, 16>), dim3((uint32_t)na), dim3(512), 0, st, ii, oo, arr_size, f);
+            hipLaunchKernelGGL((ms_kernel<U, FL, 512, 16>), dim3((uint32_t)na), dim3(512), 0, st, ii, oo, arr_size, f);
         else
             return hipErrorInvalidValue;
         hipError_t e = hipGetLastError();
@@ -62,16 +62,18 @@ static hipError_t ms_launch(uint64_t flip, uint32_t arr_size, const void *in, vo
 
 uint32_t multisort_max_size(int) { return 8192; }
 
-hipError_t multisort_run(int kb, uint64_t flip, uint32_t arr_size, const void *in, void *out, uint64_t num_arrays,
+hipError_t multisort_run(int kt, uint64_t flip, uint32_t arr_size, const void *in, void *out, uint64_t num_arrays,
                          hipStream_t st)
 {
     if (arr_size == 0 || num_arrays == 0) return hipSuccess;
-    if (arr_size > multisort_max_size(kb)) return hipErrorInvalidValue;
-    switch (kb) {
-    case 1: return ms_launch<uint8_t>(flip, arr_size, in, out, num_arrays, st);
-    case 2: return ms_launch<uint16_t>(flip, arr_size, in, out, num_arrays, st);
-    case 4: return ms_launch<uint32_t>(flip, arr_size, in, out, num_arrays, st);
-    case 8: return ms_launch<uint64_t>(flip, arr_size, in, out, num_arrays, st);
+    if (arr_size > multisort_max_size(kt)) return hipErrorInvalidValue;
+    switch (kt) {
+    case 1: return ms_launch<uint8_t, false>(flip, arr_size, in, out, num_arrays, st);
+    case 2: return ms_launch<uint16_t, false>(flip, arr_size, in, out, num_arrays, st);
+    case 4: return ms_launch<uint32_t, false>(flip, arr_size, in, out, num_arrays, st);
+    case 8: return ms_launch<uint64_t, false>(flip, arr_size, in, out, num_arrays, st);
+    case 4 | KT_FLOAT: return ms_launch<uint32_t, true>(flip, arr_size, in, out, num_arrays, st);
+    case 8 | KT_FLOAT: return ms_launch<uint64_t, true>(flip, arr_size, in, out, num_arrays, st);
     default: return hipErrorInvalidValue;
     }
 }

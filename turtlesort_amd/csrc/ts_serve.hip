@@ -72,13 +72,13 @@ struct turtle_server {
     std::mutex mu;
     std::condition_variable cv_work;   // new requests / shutdown
     std::condition_variable cv_done;   // a request finished
-    std::deque<std::shared_ptr<Request>> pending[TURTLE_UINT64 + 1];
+    std::deque<std::shared_ptr<Request>> pending[TURTLE_FLOAT64 + 1];
     std::unordered_map<uint64_t, std::shared_ptr<Request>> live;
     uint64_t next_ticket = 1;
     bool stop = false;
     int device = 0;
     turtle_server_stats st{};
-    int64_t max_keys[TURTLE_UINT64 + 1] = {0};
+    int64_t max_keys[TURTLE_FLOAT64 + 1] = {0};
 };
 
 namespace {
@@ -198,12 +198,12 @@ int turtle_server_create(const turtle_server_config *cfg, turtle_server_handle *
     if (cfg) c = *cfg;
     if (c.pool_bytes <= 0) c.pool_bytes = (int64_t)64 << 30;
     if (c.slots_per_dtype <= 0) c.slots_per_dtype = 2;
-    if (c.dtype_mask == 0) c.dtype_mask = 0x1fe;   // all eight key types
-    if (c.slots_per_dtype > 16 || (c.dtype_mask & ~0x1fe)) return TURTLE_ERR_INVALID;
+    if (c.dtype_mask == 0) c.dtype_mask = 0x7fe;   // all ten key types
+    if (c.slots_per_dtype > 16 || (c.dtype_mask & ~0x7fe)) return TURTLE_ERR_INVALID;
     // device bytes per key of a slot: 4 key buffers of the handle, 4 index
     // buffers, and the server's in/out key and index buffers
     int64_t per_key_sum = 0;
-    for (int d = TURTLE_INT8; d <= TURTLE_UINT64; d++)
+    for (int d = TURTLE_INT8; d <= TURTLE_FLOAT64; d++)
         if (c.dtype_mask & (1 << d)) per_key_sum += (int64_t)c.slots_per_dtype * (6 * key_bytes(d) + 6 * 4);
     if (c.slot_keys <= 0) {
         c.slot_keys = c.pool_bytes / per_key_sum;
@@ -215,7 +215,7 @@ int turtle_server_create(const turtle_server_config *cfg, turtle_server_handle *
     sv->cfg = c;
     (void)hipGetDevice(&sv->device);
     int rc = TURTLE_OK;
-    for (int d = TURTLE_INT8; d <= TURTLE_UINT64 && rc == TURTLE_OK; d++) {
+    for (int d = TURTLE_INT8; d <= TURTLE_FLOAT64 && rc == TURTLE_OK; d++) {
         if (!(c.dtype_mask & (1 << d))) continue;
         sv->max_keys[d] = c.slot_keys;
         for (int k = 0; k < c.slots_per_dtype && rc == TURTLE_OK; k++) {
@@ -256,7 +256,7 @@ int turtle_server_submit(turtle_server_handle sv, int dtype, void *keys, int32_t
 {
     if (!sv || !ticket) return TURTLE_ERR_INVALID;
     *ticket = 0;
-    if (dtype < TURTLE_INT8 || dtype > TURTLE_UINT64) return TURTLE_ERR_DTYPE;
+    if (dtype < TURTLE_INT8 || dtype > TURTLE_FLOAT64) return TURTLE_ERR_DTYPE;
     if (n < 0 || (n > 0 && !keys)) return TURTLE_ERR_INVALID;
     std::shared_ptr<Request> r(new Request());
     r->dtype = dtype;
