@@ -161,6 +161,21 @@ struct TurtleSort {
         return s;
     }
 
+    // MultiSort with index tracking, which the reference lists as
+    // unsupported (README.md:46): index[i] moves with hostDataToSort[i],
+    // stably; both arrays are sorted in place
+    template <typename T, int ArrSize, int BlockSize = 32, bool UseSharedMemory = false>
+    double MultiSort(const int numArraysToSort, T *hostDataToSort, int *index)
+    {
+        static_assert(std::is_same<T, Type>::value, "MultiSort element type must be the sorter's Type");
+        static_assert(ArrSize > 0 && BlockSize > 0, "ArrSize and BlockSize must be positive");
+        static_assert(TrackIndex, "MultiSort with an index needs a TurtleSort<Type, true>");
+        double s = 0.0;
+        TURTLE_CHECK(turtle_multisort_index(h, dtypeOf<Type>(), ArrSize, numArraysToSort, hostDataToSort,
+                                            reinterpret_cast<int32_t *>(index), 0, &s));
+        return s;
+    }
+
     ~TurtleSort()
     {
         if (h) turtle_destroy(h);

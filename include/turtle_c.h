@@ -110,9 +110,22 @@ int turtle_sort_sync(turtle_handle h, double *seconds);
  * sorts num_arrays consecutive arrays of arr_size keys each, in place in the
  * host buffer, synchronously; *seconds gets the wall time including
  * transfers.  block_size and use_lds are accepted for source compatibility
- * and do not change the result. */
+ * and do not change the result.  Arrays of up to 8192 keys move in chunks
+ * through two pinned buffers on two streams, so that the host copies of one
+ * chunk overlap the transfers and the sort of the other; host copies use
+ * $TURTLE_COPY_THREADS threads (default 8). */
 int turtle_multisort(turtle_handle h, int dtype, int64_t arr_size, int block_size, int use_lds,
                      int64_t num_arrays, void *host_data, double *seconds);
+
+/* MultiSort with index tracking (no reference counterpart: the reference's
+ * README.md:46 lists it as unsupported).  Sorts num_arrays consecutive
+ * arrays of arr_size keys in place in host_keys and moves a 32-bit value
+ * with every key, stably: host_idx holds the values on entry, or, when
+ * local_positions is non-zero, is output only and receives each key's
+ * position within its array (an argsort per array).  Synchronous; *seconds
+ * gets the wall time including transfers.  The handle must track indices. */
+int turtle_multisort_index(turtle_handle h, int dtype, int64_t arr_size, int64_t num_arrays, void *host_keys,
+                           int32_t *host_idx, int local_positions, double *seconds);
 
 /* ---- device-resident forms (inputs already in HBM) ---------------------- */
 
@@ -144,6 +157,15 @@ int turtle_segmented_sort_device(turtle_handle h, const void *keys_in, const int
  * Asynchronous. */
 int turtle_multisort_device(turtle_handle h, int dtype, int64_t arr_size,
                             const void *keys_in, void *keys_out, int64_t num_arrays, void *stream);
+
+/* Device form of turtle_multisort_index: idx_out (required) receives the
+ * values of idx_in moved with their keys, or each key's position within its
+ * array when idx_in is NULL.  idx buffers hold arr_size*num_arrays entries
+ * (at most 2^32).  Arrays longer than 8192 keys go through the sort engine
+ * and need a handle that tracks indices.  Asynchronous. */
+int turtle_multisort_index_device(turtle_handle h, int dtype, int64_t arr_size, const void *keys_in,
+                                  const int32_t *idx_in, void *keys_out, int32_t *idx_out, int64_t num_arrays,
+                                  void *stream);
 
 /* hipGraph form of turtle_segmented_sort_device: the reset, root, level
  * and LDS-sort launches are captured once for a fixed number of levels

@@ -6,6 +6,8 @@ import pytest
 
 from turtlesort_amd import TurtleSort, fill_keys, workloads as wl
 
+from conftest import d2h, h2d
+
 pytestmark = pytest.mark.gpu
 
 
@@ -52,4 +54,67 @@ def test_multisort_rejects_bad_sizes(torch_dev):
         sorter.multi_sort(1000, d)          # 2 x 1000 > max 1000
     with pytest.raises(TypeError):
         sorter.multi_sort(10, np.zeros(100, dtype=np.int64))
+    sorter.close()
+
+
+# ---------------------------------------------------------------- with an index
+def row_expect(orc, d, arr):
+    """per array: keys as the oracle sorts them, and the stable permutation"""
+    rows = d.reshape(-1, arr)
+    perm = np.argsort(orc.order(d).reshape(-1, arr), axis=1, kind="stable")
+    return np.take_along_axis(rows, perm, axis=1).reshape(-1), perm.astype(np.int32)
+
+
+@pytest.mark.parametrize("dtype", [np.int8, np.uint16, np.int32, np.uint64, np.float32, np.float64])
+def test_multisort_index_host(orc, torch_dev, dtype):
+    """turtle_multisort_index: keys equal the oracle's per array; the index
+    is each key's position (local) or the payload moved stably with it.
+    The 2038 x 4000 case spans several 16 MiB staging chunks (3 for 1-byte
+    keys, 6 for 8-byte keys with the index)."""
+    sorter = TurtleSort(dtype, 2038 * 4000)
+    ui = np.dtype(f"u{np.dtype(dtype).itemsize}")
+    for arr, num in [(1, 9), (7, 300), (64, 33), (2038, 4000), (8192, 5), (8193, 3), (30000, 2)]:
+        print(f"case dtype={np.dtype(dtype)} arr={arr} num={num}", flush=True)
+        d = wl.gen_keys(dtype, arr * num, arr * 3 + num, 0 if np.dtype(dtype).kind == "f" else 60)
+        ek, perm = row_expect(orc, d, arr)
+        got = d.copy()
+        li = sorter.multi_sort_index(arr, got)
+        np.testing.assert_array_equal(got.view(ui), ek.view(ui))
+        np.testing.assert_array_equal(li.reshape(-1, arr), perm)
+        got = d.copy()
+        pay = (np.arange(d.size, dtype=np.int64) * 7919 % (2**31 - 1)).astype(np.int32)
+        out = sorter.multi_sort_index(arr, got, pay.copy())
+        np.testing.assert_array_equal(got.view(ui), ek.view(ui))
+        np.testing.assert_array_equal(out.reshape(-1, arr), np.take_along_axis(pay.reshape(-1, arr), perm, axis=1))
+        # the keys-only host form through the same chunks
+        got = d.copy()
+        sorter.multi_sort(arr, got)
+        np.testing.assert_array_equal(got.view(ui), ek.view(ui))
+    sorter.close()
+
+
+@pytest.mark.parametrize("arr", [100, 1024, 8192, 9000])
+def test_multisort_index_device(orc, torch_dev, arr):
+    """device form: local positions and a payload, int64 keys with heavy
+    duplicates (stability decides the index)"""
+    torch = torch_dev
+    num = max(4, (1 << 20) // arr)
+    n = num * arr
+    print(f"case device arr={arr} num={num}", flush=True)
+    d = wl.gen_keys(np.int64, n, arr, 58)
+    ek, perm = row_expect(orc, d, arr)
+    sorter = TurtleSort(np.int64, n)
+    kin = h2d(torch, d)
+    kout = torch.empty_like(kin)
+    iout = torch.empty(n, dtype=torch.int32, device="cuda")
+    sorter.multi_sort_index_device(arr, kin, kout, iout)
+    sorter.synchronize()
+    np.testing.assert_array_equal(d2h(torch, kout), ek)
+    np.testing.assert_array_equal(d2h(torch, iout).reshape(-1, arr), perm)
+    pay = torch.arange(n, dtype=torch.int32, device="cuda") * 3
+    iout2 = torch.empty_like(iout)
+    sorter.multi_sort_index_device(arr, kin, kout, iout2, idx_in=pay)
+    sorter.synchronize()
+    np.testing.assert_array_equal(d2h(torch, iout2).reshape(-1, arr),
+                                  np.take_along_axis(d2h(torch, pay).reshape(-1, arr), perm, axis=1))
     sorter.close()

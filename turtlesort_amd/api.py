@@ -132,6 +132,27 @@ class TurtleSort:
             data.size // arr_size, data.ctypes.data_as(ctypes.c_void_p), ctypes.byref(sec)))
         return sec.value
 
+    def multi_sort_index(self, arr_size, data, idx=None):
+        """MultiSort with index tracking (turtle_multisort_index): sorts the
+        arrays in place and returns the index array moved with the keys --
+        idx's values (also in place), or, when idx is None, each key's
+        position within its array"""
+        if not isinstance(data, np.ndarray) or data.dtype != self.np_dtype or not data.flags.c_contiguous:
+            raise TypeError(f"data must be a C-contiguous numpy array of {self.np_dtype}")
+        if arr_size <= 0 or data.size % arr_size:
+            raise ValueError("data length must be a multiple of arr_size")
+        local = idx is None
+        if local:
+            idx = np.empty(data.size, dtype=np.int32)
+        elif idx.dtype != np.int32 or idx.size != data.size or not idx.flags.c_contiguous:
+            raise TypeError("idx must be a C-contiguous int32 array of the data's length")
+        sec = ctypes.c_double(0.0)
+        _lib.check(self._lib.turtle_multisort_index(
+            self._h, self.dtype, int(arr_size), data.size // arr_size, data.ctypes.data_as(ctypes.c_void_p),
+            idx.ctypes.data_as(ctypes.c_void_p), int(local), ctypes.byref(sec)))
+        self.last_seconds = sec.value
+        return idx
+
     # ---- device-resident forms (torch tensors on the GPU) --------------------------
     def sort_device(self, keys_in, keys_out, idx_in=None, idx_out=None, stream=None):
         n = keys_in.numel()
@@ -158,6 +179,12 @@ class TurtleSort:
         num = keys_in.numel() // arr_size
         _lib.check(self._lib.turtle_multisort_device(
             self._h, self.dtype, int(arr_size), _ptr(keys_in), _ptr(keys_out), num, _stream(stream)))
+
+    def multi_sort_index_device(self, arr_size, keys_in, keys_out, idx_out, idx_in=None, stream=None):
+        num = keys_in.numel() // arr_size
+        _lib.check(self._lib.turtle_multisort_index_device(
+            self._h, self.dtype, int(arr_size), _ptr(keys_in), _ptr(idx_in), _ptr(keys_out), _ptr(idx_out), num,
+            _stream(stream)))
 
     def synchronize(self):
         st = _lib.Stats()

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/turtle_c.h"
+#include "ts_copy.h"
 #include "ts_engine.h"
 #include "ts_internal.h"
 
@@ -189,6 +190,8 @@ struct turtle_sorter {
     void *stage[2] = {nullptr, nullptr};
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
     bool stage_used[2] = {false, false};
+    ts::CopyPool *pool = nullptr;        // threads of the user <-> staging copies
+    hipStream_t stream2 = nullptr;       // second stream of the chunked host MultiSort
     // what the last engine run needs to be run again after its tables grow
     bool last_engine = false;
     uint64_t last_total = 0;
@@ -540,6 +543,7 @@ int ensure_staging(turtle_sorter *h)
         if (!h->stage[k]) HIPCHK(hipHostMalloc(&h->stage[k], STAGE_BYTES, hipHostMallocDefault), "hipHostMalloc(staging)");
         if (!h->stage_ev[k]) HIPCHK(hipEventCreateWithFlags(&h->stage_ev[k], hipEventDisableTiming), "hipEventCreate");
     }
+    if (!h->pool) h->pool = new ts::CopyPool(ts::CopyPool::default_threads());
     return TURTLE_OK;
 }
 
@@ -553,7 +557,7 @@ int stage_h2d(turtle_sorter *h, void *dev, const void *host, size_t bytes, hipSt
     for (size_t off = 0; off < bytes; off += STAGE_BYTES, k ^= 1) {
         const size_t len = bytes - off < STAGE_BYTES ? bytes - off : STAGE_BYTES;
         if (h->stage_used[k]) HIPCHK(hipEventSynchronize(h->stage_ev[k]), "hipEventSynchronize");
-        memcpy(h->stage[k], (const char *)host + off, len);
+        h->pool->copy(h->stage[k], (const char *)host + off, len);
         HIPCHK(hipMemcpyAsync((char *)dev + off, h->stage[k], len, hipMemcpyHostToDevice, st), "H2D");
         HIPCHK(hipEventRecord(h->stage_ev[k], st), "hipEventRecord");
         h->stage_used[k] = true;
@@ -584,7 +588,7 @@ int stage_d2h(turtle_sorter *h, void *host, const void *dev, size_t bytes, hipSt
         const size_t off = i * STAGE_BYTES;
         const size_t len = bytes - off < STAGE_BYTES ? bytes - off : STAGE_BYTES;
         HIPCHK(hipEventSynchronize(h->stage_ev[k]), "hipEventSynchronize");
-        memcpy((char *)host + off, h->stage[k], len);
+        h->pool->copy((char *)host + off, h->stage[k], len);
     }
     return TURTLE_OK;
 }
@@ -696,7 +700,9 @@ int turtle_destroy(turtle_handle h)
     if (!h) return TURTLE_OK;
     if (h->last_stream != h->stream) (void)hipStreamSynchronize(h->last_stream);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (h->stream2) (void)hipStreamSynchronize(h->stream2);
     delete h->timer;
+    delete h->pool;
     void *bufs[] = {h->kin, h->kout, h->kt0, h->kt1, h->iin, h->iout, h->it0, h->it1, h->bins, h->ctrl, h->dev_off};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -710,6 +716,7 @@ int turtle_destroy(turtle_handle h)
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);
+    if (h->stream2) (void)hipStreamDestroy(h->stream2);
     delete h;
     return TURTLE_OK;
 }
@@ -886,8 +893,12 @@ int turtle_segmented_sort_device(turtle_handle h, const void *keys_in, const int
                       h->dev_off, (uint32_t)num_segments, st);
 }
 
-int turtle_multisort_device(turtle_handle h, int dtype, int64_t arr_size, const void *keys_in, void *keys_out,
-                            int64_t num_arrays, void *stream)
+// MultiSort on device buffers, with an optional 32-bit payload (idx_out
+// non-NULL: idx_in's values, or each key's position in its array when
+// idx_in is NULL).  Arrays up to multisort_max_size keys: one workgroup per
+// array; longer ones: equal segments through the sort engine.
+static int multisort_device(turtle_handle h, int dtype, int64_t arr_size, const void *keys_in, const int32_t *idx_in,
+                            void *keys_out, int32_t *idx_out, int64_t num_arrays, void *stream)
 {
     if (!h) return fail(TURTLE_ERR_INVALID, "NULL handle");
     if (dtype != h->dtype) return fail(TURTLE_ERR_DTYPE, "dtype differs from the handle's");
@@ -896,10 +907,14 @@ int turtle_multisort_device(turtle_handle h, int dtype, int64_t arr_size, const 
     if (!keys_in || !keys_out) return fail(TURTLE_ERR_INVALID, "NULL key buffer");
     if (arr_size >= (int64_t)1 << 31 || num_arrays > ((int64_t)1 << 62) / arr_size)
         return fail(TURTLE_ERR_CAPACITY, "batch too large");
+    const bool idx = idx_out != nullptr;
+    if (idx && (uint64_t)arr_size * (uint64_t)num_arrays > 0xffffffffull)
+        return fail(TURTLE_ERR_CAPACITY, "an indexed batch is limited to 2^32 keys");
     hipStream_t st = stream_of(h, stream);
-    if ((uint64_t)arr_size <= ts::multisort_max_size(h->kb)) {
+    if ((uint64_t)arr_size <= ts::multisort_max_size(h->kt)) {
         HIPCHK(hipEventRecord(h->ev0, st), "hipEventRecord");
-        HIPCHK(ts::multisort_run(h->kt, h->flip, (uint32_t)arr_size, keys_in, keys_out, (uint64_t)num_arrays, st),
+        HIPCHK(ts::multisort_run(h->kt, h->flip, (uint32_t)arr_size, keys_in, keys_out, (const uint32_t *)idx_in,
+                                 (uint32_t *)idx_out, (uint64_t)num_arrays, st),
                "multisort launch");
         HIPCHK(hipEventRecord(h->ev1, st), "hipEventRecord");
         h->timed = true;
@@ -911,9 +926,30 @@ int turtle_multisort_device(turtle_handle h, int dtype, int64_t arr_size, const 
     // longer arrays: equal segments through the sort engine
     const int64_t total = arr_size * num_arrays;
     if (total > h->max_n) return fail(TURTLE_ERR_CAPACITY, "arrays exceed the handle's max_n");
+    if (idx && !h->track) return fail(TURTLE_ERR_INVALID, "the handle does not track indices");
+    if (idx && !idx_in) {
+        // positions inside each array, sorted in place
+        HIPCHK(ts::fill_iota((uint32_t *)idx_out, (uint64_t)total, st, (uint64_t)arr_size), "fill_iota");
+        idx_in = idx_out;
+    }
     std::vector<int64_t> off((size_t)num_arrays + 1);
     for (int64_t a = 0; a <= num_arrays; a++) off[a] = a * arr_size;
-    return turtle_segmented_sort_device(h, keys_in, nullptr, keys_out, nullptr, off.data(), num_arrays, stream);
+    return turtle_segmented_sort_device(h, keys_in, idx ? idx_in : nullptr, keys_out, idx_out, off.data(), num_arrays,
+                                        stream);
+}
+
+int turtle_multisort_device(turtle_handle h, int dtype, int64_t arr_size, const void *keys_in, void *keys_out,
+                            int64_t num_arrays, void *stream)
+{
+    return multisort_device(h, dtype, arr_size, keys_in, nullptr, keys_out, nullptr, num_arrays, stream);
+}
+
+int turtle_multisort_index_device(turtle_handle h, int dtype, int64_t arr_size, const void *keys_in,
+                                  const int32_t *idx_in, void *keys_out, int32_t *idx_out, int64_t num_arrays,
+                                  void *stream)
+{
+    if (!idx_out) return fail(TURTLE_ERR_INVALID, "NULL index output");
+    return multisort_device(h, dtype, arr_size, keys_in, idx_in, keys_out, idx_out, num_arrays, stream);
 }
 
 int turtle_synchronize(turtle_handle h, turtle_stats *st)
@@ -980,6 +1016,90 @@ int turtle_sort_sync(turtle_handle h, double *seconds)
     return TURTLE_OK;
 }
 
+// Host MultiSort.  Arrays that one workgroup sorts go in chunks of whole
+// arrays through the two pinned staging slots: chunk c is copied into slot
+// c&1 by the copy threads, then its H2D copy, sort and D2H copy are queued
+// on stream c&1, and the slot's previous chunk (c-2) is copied out to the
+// user's buffer first.  So the host copies of one chunk overlap the
+// transfers and the sort of the other, and the two streams' H2D and D2H
+// copies overlap on the link.  Longer arrays: one staged batch through the
+// sort engine.  idx != NULL: a payload per key (local: output only, each
+// key's position in its array); the handle must track indices.
+static int multisort_host(turtle_handle h, int64_t arr_size, int64_t num_arrays, void *keys, int32_t *idx,
+                          bool local)
+{
+    const int64_t total = arr_size * num_arrays;
+    if (total == 0) return TURTLE_OK;
+    if (!keys || (local && !idx)) return fail(TURTLE_ERR_INVALID, "NULL host data");
+    if (idx && !h->track) return fail(TURTLE_ERR_INVALID, "the handle does not track indices");
+    const size_t kb = (size_t)h->kb, ib = idx ? 4u : 0u;
+    const size_t kalign = 256;
+    const size_t per_arr = (size_t)arr_size * (kb + ib) + kalign;
+    int rc = ensure_staging(h);
+    if (rc) return rc;
+    if ((uint64_t)arr_size > ts::multisort_max_size(h->kt) || per_arr > STAGE_BYTES) {
+        if ((rc = stage_h2d(h, h->kin, keys, (size_t)total * kb, h->stream))) return rc;
+        if (idx && !local && (rc = stage_h2d(h, h->iin, idx, (size_t)total * 4, h->stream))) return rc;
+        rc = multisort_device(h, h->dtype, arr_size, h->kin, idx && !local ? (const int32_t *)h->iin : nullptr,
+                              h->kout, idx ? (int32_t *)h->iout : nullptr, num_arrays, h->stream);
+        if (rc) return rc;
+        if ((rc = turtle_synchronize(h, nullptr))) return rc;
+        if ((rc = stage_d2h(h, keys, h->kout, (size_t)total * kb, h->stream))) return rc;
+        if (idx && (rc = stage_d2h(h, idx, h->iout, (size_t)total * 4, h->stream))) return rc;
+        return TURTLE_OK;
+    }
+    if (!h->stream2) HIPCHK(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking), "hipStreamCreate");
+    hipStream_t sts[2] = {h->stream, h->stream2};
+    // order both streams after the handle's earlier work, and the slots
+    // after their last transfers
+    HIPCHK(hipStreamSynchronize(h->last_stream), "hipStreamSynchronize");
+    for (int k = 0; k < 2; k++)
+        if (h->stage_used[k]) HIPCHK(hipEventSynchronize(h->stage_ev[k]), "hipEventSynchronize");
+    const int64_t cha = (int64_t)((STAGE_BYTES - kalign) / ((size_t)arr_size * (kb + ib)));   // arrays per chunk
+    const int64_t nch = (num_arrays + cha - 1) / cha;
+    int64_t ch_arrays[2] = {0, 0}, ch_first[2] = {0, 0};
+    auto idx_off = [&](int64_t nk) { return ((size_t)nk * kb + kalign - 1) & ~(kalign - 1); };
+    auto retire = [&](int k) -> int {
+        HIPCHK(hipEventSynchronize(h->stage_ev[k]), "hipEventSynchronize");
+        const int64_t nk = ch_arrays[k] * arr_size, e0 = ch_first[k] * arr_size;
+        const char *slot = (const char *)h->stage[k];
+        h->pool->copy((char *)keys + (size_t)e0 * kb, slot, (size_t)nk * kb);
+        if (idx) h->pool->copy(idx + e0, slot + idx_off(nk), (size_t)nk * 4);
+        ch_arrays[k] = 0;
+        return TURTLE_OK;
+    };
+    for (int64_t c = 0; c < nch; c++) {
+        const int k = (int)(c & 1);
+        if (ch_arrays[k] && (rc = retire(k))) return rc;
+        const int64_t a0 = c * cha, na = num_arrays - a0 < cha ? num_arrays - a0 : cha;
+        const int64_t nk = na * arr_size, e0 = a0 * arr_size;
+        char *slot = (char *)h->stage[k];
+        h->pool->copy(slot, (const char *)keys + (size_t)e0 * kb, (size_t)nk * kb);
+        if (idx && !local) h->pool->copy(slot + idx_off(nk), idx + e0, (size_t)nk * 4);
+        void *dkin = (char *)h->kin + (size_t)e0 * kb, *dkout = (char *)h->kout + (size_t)e0 * kb;
+        uint32_t *diin = idx ? h->iin + e0 : nullptr, *diout = idx ? h->iout + e0 : nullptr;
+        hipStream_t st = sts[k];
+        HIPCHK(hipMemcpyAsync(dkin, slot, (size_t)nk * kb, hipMemcpyHostToDevice, st), "H2D");
+        if (idx && !local) HIPCHK(hipMemcpyAsync(diin, slot + idx_off(nk), (size_t)nk * 4, hipMemcpyHostToDevice, st), "H2D");
+        HIPCHK(ts::multisort_run(h->kt, h->flip, (uint32_t)arr_size, dkin, dkout, idx && !local ? diin : nullptr,
+                                 diout, (uint64_t)na, st),
+               "multisort launch");
+        HIPCHK(hipMemcpyAsync(slot, dkout, (size_t)nk * kb, hipMemcpyDeviceToHost, st), "D2H");
+        if (idx) HIPCHK(hipMemcpyAsync(slot + idx_off(nk), diout, (size_t)nk * 4, hipMemcpyDeviceToHost, st), "D2H");
+        HIPCHK(hipEventRecord(h->stage_ev[k], st), "hipEventRecord");
+        h->stage_used[k] = true;
+        ch_arrays[k] = na;
+        ch_first[k] = a0;
+    }
+    for (int64_t c = nch; c < nch + 2; c++) {
+        const int k = (int)(c & 1);
+        if (ch_arrays[k] && (rc = retire(k))) return rc;
+    }
+    h->timed = false;   // no single device interval to report
+    h->last_stream = h->stream;
+    return TURTLE_OK;
+}
+
 int turtle_multisort(turtle_handle h, int dtype, int64_t arr_size, int block_size, int use_lds, int64_t num_arrays,
                      void *host_data, double *seconds)
 {
@@ -991,17 +1111,24 @@ int turtle_multisort(turtle_handle h, int dtype, int64_t arr_size, int block_siz
     if (arr_size < 0 || num_arrays < 0) return fail(TURTLE_ERR_INVALID, "negative size");
     if (arr_size > 0 && num_arrays > h->max_n / (arr_size ? arr_size : 1))
         return fail(TURTLE_ERR_CAPACITY, "numArrays*arrSize exceeds the handle's max_n");
-    const int64_t total = arr_size * num_arrays;
-    if (total > 0) {
-        if (!host_data) return fail(TURTLE_ERR_INVALID, "NULL host data");
-        int rc = stage_h2d(h, h->kin, host_data, (size_t)total * h->kb, h->stream);
-        if (rc) return rc;
-        rc = turtle_multisort_device(h, dtype, arr_size, h->kin, h->kout, num_arrays, h->stream);
-        if (rc) return rc;
-        rc = turtle_synchronize(h, nullptr);
-        if (rc) return rc;
-        if ((rc = stage_d2h(h, host_data, h->kout, (size_t)total * h->kb, h->stream))) return rc;
-    }
+    int rc = multisort_host(h, arr_size, num_arrays, host_data, nullptr, false);
+    if (rc) return rc;
+    if (seconds) *seconds = now_s() - t0;
+    return TURTLE_OK;
+}
+
+int turtle_multisort_index(turtle_handle h, int dtype, int64_t arr_size, int64_t num_arrays, void *host_keys,
+                           int32_t *host_idx, int local_positions, double *seconds)
+{
+    if (!h) return fail(TURTLE_ERR_INVALID, "NULL handle");
+    const double t0 = now_s();
+    if (dtype != h->dtype) return fail(TURTLE_ERR_DTYPE, "dtype differs from the handle's");
+    if (arr_size < 0 || num_arrays < 0) return fail(TURTLE_ERR_INVALID, "negative size");
+    if (arr_size > 0 && num_arrays > h->max_n / (arr_size ? arr_size : 1))
+        return fail(TURTLE_ERR_CAPACITY, "numArrays*arrSize exceeds the handle's max_n");
+    if (arr_size * num_arrays > 0 && !host_idx) return fail(TURTLE_ERR_INVALID, "NULL host index");
+    int rc = multisort_host(h, arr_size, num_arrays, host_keys, host_idx, local_positions != 0);
+    if (rc) return rc;
     if (seconds) *seconds = now_s() - t0;
     return TURTLE_OK;
 }

@@ -28,14 +28,16 @@ hipError_t engine_levels(const Params &p, int kt, bool idx, int splitters, int c
                          bool first, bool last, hipStream_t st, LaunchTimer *timer = nullptr);
 
 // MultiSort kernels (ts_multisort.hip): arrays of arr_size keys, in -> out.
-// kt: key type code.  Returns hipErrorInvalidValue when arr_size exceeds
+// kt: key type code.  iout != nullptr: a 32-bit payload moves with every
+// key, read from iin, or each key's position in its array when iin is
+// nullptr.  Returns hipErrorInvalidValue when arr_size exceeds
 // multisort_max_size(kt).
-hipError_t multisort_run(int kt, uint64_t flip, uint32_t arr_size, const void *in, void *out,
-                         uint64_t num_arrays, hipStream_t st);
+hipError_t multisort_run(int kt, uint64_t flip, uint32_t arr_size, const void *in, void *out, const uint32_t *iin,
+                         uint32_t *iout, uint64_t num_arrays, hipStream_t st);
 uint32_t multisort_max_size(int kb);
 
 // synthetic inputs (ts_util.hip)
 hipError_t fill_keys(int kb, void *keys, uint64_t n, uint64_t seed, int shift, hipStream_t st);
-hipError_t fill_iota(uint32_t *idx, uint64_t n, hipStream_t st);
+hipError_t fill_iota(uint32_t *idx, uint64_t n, hipStream_t st, uint64_t period = 0);   // period > 0: i mod period
 
 }  // namespace ts

@@ -14,25 +14,35 @@
 
 namespace ts {
 
-template <typename U, bool FL, int NT, int ITEMS>
-__global__ void __launch_bounds__(NT) ms_kernel(const U *__restrict__ in, U *__restrict__ out, uint32_t arr_size,
-                                                 U flip)
+// IDX: a 32-bit payload per key moves with it (stably); iin == nullptr
+// gives each key its position within its array.
+template <typename U, bool FL, bool IDX, int NT, int ITEMS>
+__global__ void __launch_bounds__(NT) ms_kernel(const U *__restrict__ in, U *__restrict__ out,
+                                                 const uint32_t *__restrict__ iin, uint32_t *__restrict__ iout,
+                                                 uint32_t arr_size, U flip)
 {
     constexpr uint32_t CAPA = (uint32_t)NT * ITEMS;
     __shared__ U sk[padded_len<ITEMS>(CAPA)];
+    __shared__ uint32_t si[IDX ? padded_len<ITEMS>(CAPA) : 1];
     const size_t base = (size_t)blockIdx.x * arr_size;
     uint32_t n_eff = next_pow2_dev(arr_size);
     if (n_eff < (uint32_t)ITEMS) n_eff = ITEMS;
-    for (uint32_t i = threadIdx.x; i < n_eff; i += NT)
-        sk[pidx<ITEMS>(i)] = i < arr_size ? to_ord<U, FL>(in[base + i], flip) : key_max<U>();
+    for (uint32_t i = threadIdx.x; i < n_eff; i += NT) {
+        const bool real = i < arr_size;
+        sk[pidx<ITEMS>(i)] = real ? to_ord<U, FL>(in[base + i], flip) : key_max<U>();
+        if (IDX) si[pidx<ITEMS>(i)] = real && iin ? iin[base + i] : i;
+    }
     block_sync();
-    block_sort<U, false, NT, ITEMS>(sk, nullptr, n_eff, n_eff);
-    for (uint32_t i = threadIdx.x; i < arr_size; i += NT) out[base + i] = from_ord<U, FL>(sk[pidx<ITEMS>(i)], flip);
+    block_sort<U, IDX, NT, ITEMS>(sk, si, n_eff, n_eff);
+    for (uint32_t i = threadIdx.x; i < arr_size; i += NT) {
+        out[base + i] = from_ord<U, FL>(sk[pidx<ITEMS>(i)], flip);
+        if (IDX) iout[base + i] = si[pidx<ITEMS>(i)];
+    }
 }
 
-template <typename U, bool FL>
-static hipError_t ms_launch(uint64_t flip, uint32_t arr_size, const void *in, void *out, uint64_t num_arrays,
-                            hipStream_t st)
+template <typename U, bool FL, bool IDX>
+static hipError_t ms_launch(uint64_t flip, uint32_t arr_size, const void *in, void *out, const uint32_t *iin,
+                            uint32_t *iout, uint64_t num_arrays, hipStream_t st)
 {
     const U f = (U)flip;
     const U *i = (const U *)in;
@@ -42,16 +52,23 @@ static hipError_t ms_launch(uint64_t flip, uint32_t arr_size, const void *in, vo
     const uint64_t chunk = 1ull << 30;
     for (uint64_t a0 = 0; a0 < num_arrays; a0 += chunk) {
         const uint64_t na = (num_arrays - a0) < chunk ? (num_arrays - a0) : chunk;
-        const U *ii = i + a0 * arr_size;
-        U *oo = o + a0 * arr_size;
+        const uint64_t e0 = a0 * arr_size;
+        const U *ii = i + e0;
+        U *oo = o + e0;
+        const uint32_t *xi = iin ? iin + e0 : nullptr;
+        uint32_t *xo = IDX ? iout + e0 : nullptr;
         if (ne <= 256)
-            hipLaunchKernelGGL((ms_kernel<U, FL, 64, 4>), dim3((uint32_t)na), dim3(64), 0, st, ii, oo, arr_size, f);
+            hipLaunchKernelGGL((ms_kernel<U, FL, IDX, 64, 4>), dim3((uint32_t)na), dim3(64), 0, st, ii, oo, xi, xo,
+                               arr_size, f);
         else if (ne <= 1024)
-            hipLaunchKernelGGL((ms_kernel<U, FL, 128, 8>), dim3((uint32_t)na), dim3(128), 0, st, ii, oo, arr_size, f);
+            hipLaunchKernelGGL((ms_kernel<U, FL, IDX, 128, 8>), dim3((uint32_t)na), dim3(128), 0, st, ii, oo, xi, xo,
+                               arr_size, f);
         else if (ne <= 4096)
-            hipLaunchKernelGGL((ms_kernel<U, FL, 256, 16>), dim3((uint32_t)na), dim3(256), 0, st, ii, oo, arr_size, f);
+            hipLaunchKernelGGL((ms_kernel<U, FL, IDX, 256, 16>), dim3((uint32_t)na), dim3(256), 0, st, ii, oo, xi,
+                               xo, arr_size, f);
         else if (ne <= 8192)
-            hipLaunchKernelGGL((ms_kernel<U, FL, 512, 16>), dim3((uint32_t)na), dim3(512), 0, st, ii, oo, arr_size, f);
+            hipLaunchKernelGGL((ms_kernel<U, FL, IDX, 512, 16>), dim3((uint32_t)na), dim3(512), 0, st, ii, oo, xi,
+                               xo, arr_size, f);
         else
             return hipErrorInvalidValue;
         hipError_t e = hipGetLastError();
@@ -60,20 +77,28 @@ static hipError_t ms_launch(uint64_t flip, uint32_t arr_size, const void *in, vo
     return hipSuccess;
 }
 
+template <typename U, bool FL>
+static hipError_t ms_pick(uint64_t flip, uint32_t arr_size, const void *in, void *out, const uint32_t *iin,
+                          uint32_t *iout, uint64_t num_arrays, hipStream_t st)
+{
+    return iout ? ms_launch<U, FL, true>(flip, arr_size, in, out, iin, iout, num_arrays, st)
+                : ms_launch<U, FL, false>(flip, arr_size, in, out, nullptr, nullptr, num_arrays, st);
+}
+
 uint32_t multisort_max_size(int) { return 8192; }
 
-hipError_t multisort_run(int kt, uint64_t flip, uint32_t arr_size, const void *in, void *out, uint64_t num_arrays,
-                         hipStream_t st)
+hipError_t multisort_run(int kt, uint64_t flip, uint32_t arr_size, const void *in, void *out, const uint32_t *iin,
+                         uint32_t *iout, uint64_t num_arrays, hipStream_t st)
 {
     if (arr_size == 0 || num_arrays == 0) return hipSuccess;
     if (arr_size > multisort_max_size(kt)) return hipErrorInvalidValue;
     switch (kt) {
-    case 1: return ms_launch<uint8_t, false>(flip, arr_size, in, out, num_arrays, st);
-    case 2: return ms_launch<uint16_t, false>(flip, arr_size, in, out, num_arrays, st);
-    case 4: return ms_launch<uint32_t, false>(flip, arr_size, in, out, num_arrays, st);
-    case 8: return ms_launch<uint64_t, false>(flip, arr_size, in, out, num_arrays, st);
-    case 4 | KT_FLOAT: return ms_launch<uint32_t, true>(flip, arr_size, in, out, num_arrays, st);
-    case 8 | KT_FLOAT: return ms_launch<uint64_t, true>(flip, arr_size, in, out, num_arrays, st);
+    case 1: return ms_pick<uint8_t, false>(flip, arr_size, in, out, iin, iout, num_arrays, st);
+    case 2: return ms_pick<uint16_t, false>(flip, arr_size, in, out, iin, iout, num_arrays, st);
+    case 4: return ms_pick<uint32_t, false>(flip, arr_size, in, out, iin, iout, num_arrays, st);
+    case 8: return ms_pick<uint64_t, false>(flip, arr_size, in, out, iin, iout, num_arrays, st);
+    case 4 | KT_FLOAT: return ms_pick<uint32_t, true>(flip, arr_size, in, out, iin, iout, num_arrays, st);
+    case 8 | KT_FLOAT: return ms_pick<uint64_t, true>(flip, arr_size, in, out, iin, iout, num_arrays, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -23,10 +23,13 @@ __global__ void fill_keys_kernel(U *keys, uint64_t n, uint64_t seed, int shift)
         keys[i] = (U)(splitmix_at(seed, i) >> shift);
 }
 
-__global__ void fill_iota_kernel(uint32_t *idx, uint64_t n)
+// idx[i] = i, or i mod period (the position inside equal arrays of
+// `period` elements) when period > 0
+__global__ void fill_iota_kernel(uint32_t *idx, uint64_t n, uint64_t period)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) idx[i] = (uint32_t)i;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        idx[i] = (uint32_t)(period ? i % period : i);
 }
 
 static uint32_t grid_for(uint64_t n)
@@ -50,10 +53,10 @@ hipError_t fill_keys(int kb, void *keys, uint64_t n, uint64_t seed, int shift, h
     return hipGetLastError();
 }
 
-hipError_t fill_iota(uint32_t *idx, uint64_t n, hipStream_t st)
+hipError_t fill_iota(uint32_t *idx, uint64_t n, hipStream_t st, uint64_t period)
 {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(fill_iota_kernel, dim3(grid_for(n)), dim3(256), 0, st, idx, n);
+    hipLaunchKernelGGL(fill_iota_kernel, dim3(grid_for(n)), dim3(256), 0, st, idx, n, period);
     return hipGetLastError();
 }
 
