@@ -158,7 +158,8 @@ struct turtle_sorter {
     // key / index buffers of max_n elements
     void *kin = nullptr, *kout = nullptr, *kt0 = nullptr, *kt1 = nullptr;
     uint32_t *iin = nullptr, *iout = nullptr, *it0 = nullptr, *it1 = nullptr;
-    uint8_t *bins = nullptr;   // per-key bin of the current level (count -> scatter)
+    uint8_t *bins = nullptr;   // per-key bin of the current level (count -> scatter), per tile
+    uint32_t bins_tiles = 0;   // tiles the bins buffer holds
     // engine tables
     ts::Ctrl *ctrl = nullptr;
     ts::Ctrl *ctrl_host = nullptr;   // pinned readback copy
@@ -205,10 +206,12 @@ namespace {
 
 void free_tables(turtle_sorter *h)
 {
-    void *bufs[] = {h->segs, h->tcnt, h->tpre, h->spart, h->tile_seg, h->ldst, h->ldsq, h->levels};
+    void *bufs[] = {h->segs, h->tcnt, h->tpre, h->spart, h->tile_seg, h->ldst, h->ldsq, h->levels, h->bins};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     h->segs = nullptr;
+    h->bins = nullptr;
+    h->bins_tiles = 0;
     h->tcnt = h->tpre = h->spart = nullptr;
     h->tile_seg = nullptr;
     h->ldst = nullptr;
@@ -236,10 +239,16 @@ int ensure_tables(turtle_sorter *h, uint64_t total, uint32_t nroots, uint32_t sc
     if (force) segcap = force;
     const uint64_t tilecap = (total / tile + 1) + segcap;
     const uint64_t ldscap = segcap * (S + 1) + nroots + 1024;
+    // tiles of one level: every segment of a level is longer than the
+    // cutoff and the segments are disjoint, so there are at most
+    // total/TILE + total/(cutoff+1) + 1 of them; the bins buffer holds TILE
+    // bytes for each
+    uint64_t bins_tiles = total / tile + total / (cutoff + 1) + 2;
+    if (bins_tiles > tilecap) bins_tiles = tilecap;
     if (segcap > 0xffffff00ull || tilecap > 0xffffff00ull || ldscap > 0xffffff00ull)
         return fail(TURTLE_ERR_CAPACITY, "work tables would exceed 2^32 entries");
     if (h->segs && h->segcap >= segcap && h->tilecap >= tilecap && h->ldscap >= ldscap &&
-        (!force || h->segcap == segcap))
+        h->bins_tiles >= bins_tiles && (!force || h->segcap == segcap))
         return TURTLE_OK;
     // the old tables may still be in use by a run in flight
     if (h->segs) HIPCHK(hipDeviceSynchronize(), "hipDeviceSynchronize");
@@ -253,6 +262,8 @@ int ensure_tables(turtle_sorter *h, uint64_t total, uint32_t nroots, uint32_t sc
     HIPCHK(hipMalloc((void **)&h->ldst, ldscap * sizeof(ts::LdsTask)), "hipMalloc(lds tasks)");
     HIPCHK(hipMalloc((void **)&h->ldsq, ldscap * sizeof(uint32_t)), "hipMalloc(lds order)");
     HIPCHK(hipMalloc((void **)&h->levels, ts::MAX_LEVELS * sizeof(ts::Level)), "hipMalloc(levels)");
+    HIPCHK(hipMalloc((void **)&h->bins, bins_tiles * tile), "hipMalloc(bins)");
+    h->bins_tiles = (uint32_t)bins_tiles;
     h->segcap = (uint32_t)segcap;
     h->tilecap = (uint32_t)tilecap;
     h->ldscap = (uint32_t)ldscap;
@@ -283,6 +294,7 @@ ts::Params make_params(turtle_sorter *h, const void *kin, const uint32_t *iin, v
     p.segcap = h->segcap;
     p.tilecap = h->tilecap;
     p.ldscap = h->ldscap;
+    p.bins_tiles = h->bins_tiles;
     p.cutoff = (uint32_t)h->tune.lds_cutoff;
     p.tiles_per_wg = (uint32_t)(h->tune.tiles_per_task > 0 ? h->tune.tiles_per_task : 1);
     p.n_total = 0;   // set by run_engine
@@ -661,8 +673,7 @@ int turtle_create(int dtype, int64_t max_n, int track_index, int compress, turtl
     h->cus = pr.multiProcessorCount;
     const size_t kbytes = (size_t)max_n * kb, ibytes = (size_t)max_n * 4;
     if ((rc = alloc_buf(&h->kin, kbytes, "hipMalloc(keys in)")) || (rc = alloc_buf(&h->kout, kbytes, "hipMalloc(keys out)")) ||
-        (rc = alloc_buf(&h->kt0, kbytes, "hipMalloc(keys tmp0)")) || (rc = alloc_buf(&h->kt1, kbytes, "hipMalloc(keys tmp1)")) ||
-        (rc = alloc_buf((void **)&h->bins, (size_t)max_n, "hipMalloc(bins)"))) {
+        (rc = alloc_buf(&h->kt0, kbytes, "hipMalloc(keys tmp0)")) || (rc = alloc_buf(&h->kt1, kbytes, "hipMalloc(keys tmp1)"))) {
         turtle_destroy(h);
         return rc;
     }
@@ -703,7 +714,7 @@ int turtle_destroy(turtle_handle h)
     if (h->stream2) (void)hipStreamSynchronize(h->stream2);
     delete h->timer;
     delete h->pool;
-    void *bufs[] = {h->kin, h->kout, h->kt0, h->kt1, h->iin, h->iout, h->it0, h->it1, h->bins, h->ctrl, h->dev_off};
+    void *bufs[] = {h->kin, h->kout, h->kt0, h->kt1, h->iin, h->iout, h->it0, h->it1, h->ctrl, h->dev_off};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     free_tables(h);

@@ -95,9 +95,11 @@ struct Params {
     uint32_t *tile_seg;       // segment id of every tile of the current level
     LdsTask *ldst;
     uint32_t *ldsq;           // LDS tasks ordered by size class
-    uint8_t *bins;            // bin of every key of the current level (count -> scatter)
+    uint8_t *bins;            // bin of every key of the current level (count -> scatter),
+                              // TILE bytes per tile of the level (see bin_slot)
     Level *levels;
-    uint32_t segcap, tilecap, ldscap, pad0;
+    uint32_t segcap, tilecap, ldscap;
+    uint32_t bins_tiles;      // tiles the bins buffer holds
     uint32_t cutoff;          // LDS sort at or below this length
     uint32_t tiles_per_wg;    // tiles a workgroup takes per step of the tile loops
     uint32_t n_total;         // length of every key / index buffer

@@ -179,6 +179,47 @@ __device__ __forceinline__ uint32_t classify(U x, const Split<U, S> &sp, const U
     return eq ? 2 * lt + 1 : 2 * lt;
 }
 
+// classify of N keys at once, level by level: the N LDS loads of one tree
+// level are independent, so they are issued together and wait once
+template <typename U, int S, int LOG, int N>
+__device__ __forceinline__ void classify_n(const U (&x)[N], uint32_t (&bin)[N], const Split<U, S> &sp,
+                                           const U (&reg)[S <= 7 ? S : 1], const U (&reg3)[3])
+{
+    if constexpr (S <= 7) {
+#pragma unroll
+        for (int c = 0; c < N; c++) bin[c] = classify<U, S, LOG>(x[c], sp, reg, reg3);
+    } else {
+        const U r1 = reg3[0], r2 = reg3[1], r3 = reg3[2];
+        uint32_t i[N];
+#pragma unroll
+        for (int c = 0; c < N; c++) {
+            const uint32_t c1 = r1 < x[c] ? 1u : 0u;
+            const U t = c1 ? r3 : r2;
+            i[c] = 4 + 2 * c1 + (t < x[c] ? 1u : 0u);
+        }
+#pragma unroll
+        for (int l = 2; l < LOG; l++) {
+            U tv[N];
+#pragma unroll
+            for (int c = 0; c < N; c++) tv[c] = sp.tree[i[c]];
+#pragma unroll
+            for (int c = 0; c < N; c++) i[c] = 2 * i[c] + (tv[c] < x[c] ? 1u : 0u);
+        }
+        U sv[N];
+#pragma unroll
+        for (int c = 0; c < N; c++) {
+            const uint32_t lt = i[c] - (S + 1);
+            sv[c] = sp.srt[lt < (uint32_t)S ? lt : 0u];
+        }
+#pragma unroll
+        for (int c = 0; c < N; c++) {
+            const uint32_t lt = i[c] - (S + 1);
+            const bool eq = lt < (uint32_t)S && sv[c] == x[c];
+            bin[c] = eq ? 2 * lt + 1 : 2 * lt;
+        }
+    }
+}
+
 // ---- set-up kernels ------------------------------------------------------------------
 __global__ void k_reset(Ctrl *c);
 __global__ void k_roots(Params p, const uint32_t *off, uint32_t nseg, uint32_t single_len);
@@ -240,6 +281,17 @@ struct LevelK {
         }
     }
 
+    // Bins buffer: tile g of the level owns bytes [g*TILE, (g+1)*TILE).  The
+    // tile kernels give thread (w, lane) the items e = w*TI*WAVE + c*WAVE +
+    // lane (c < TI); the bins of its items c = 4q..4q+3 are the four bytes
+    // (low first) of the aligned dword at this offset, so the count pass
+    // stores and the scatter loads one dword per four items.
+    static __device__ __forceinline__ uint32_t bin_dword(uint32_t w, uint32_t q, uint32_t ln)
+    {
+        return w * (TI * WAVE) + q * (4 * WAVE) + ln * 4;
+    }
+    static_assert(TI % 4 == 0, "bins are packed four to a dword");
+
     // segment id + tile of global tile g, validated
     static __device__ __forceinline__ bool tile_of(const Params &p, uint32_t g, uint32_t &sid, uint32_t &t)
     {
@@ -278,7 +330,11 @@ struct LevelK {
                 }
                 load_reg(sh.sp, reg, reg3);
                 const U *src = src_keys<U>(p, sg->depth) + sg->begin;
-                uint8_t *bout = p.bins + sg->begin;
+                if (g >= p.bins_tiles) {
+                    if (threadIdx.x == 0) range_fail(p, 12);
+                    continue;
+                }
+                uint32_t *bout = (uint32_t *)(p.bins + (size_t)g * TILE);
                 const uint32_t ts = t * TILE;
                 const uint32_t tl = (sg->len - ts) < TILE ? (sg->len - ts) : TILE;
                 nbytes += (uint64_t)tl * (KB + 1);
@@ -295,15 +351,20 @@ struct LevelK {
                 // lanes of equal bins are serialised by the LDS
                 uint32_t bins[TI];
 #pragma unroll
-                for (int c = 0; c < TI; c++) bins[c] = classify<U, S, LOG>(to_ord<U, FL>(x[c], flip), sh.sp, reg, reg3);
+                for (int c = 0; c < TI; c++) x[c] = to_ord<U, FL>(x[c], flip);
+                classify_n<U, S, LOG, TI>(x, bins, sh.sp, reg, reg3);
 #pragma unroll
                 for (int c = 0; c < TI; c++) {
                     const uint32_t e = (uint32_t)w * (TI * WAVE) + c * WAVE + lane();
-                    if (e < tl) {
-                        atomicAdd(&sh.wcnt[w][bins[c]], 1u);
-                        bout[ts + e] = (uint8_t)bins[c];   // the scatter reads the bin back
-                    }
+                    if (e < tl) atomicAdd(&sh.wcnt[w][bins[c]], 1u);
+                    else bins[c] = NB;
                 }
+                // the scatter reads the bins back (items past the tile's end
+                // hold the marker NB)
+#pragma unroll
+                for (int q = 0; q < TI / 4; q++)
+                    bout[bin_dword(w, q, lane()) / 4] = bins[4 * q] | (bins[4 * q + 1] << 8) | (bins[4 * q + 2] << 16) |
+                                                        (bins[4 * q + 3] << 24);
                 block_sync();
                 if (threadIdx.x < (uint32_t)NB) {
                     uint32_t s = 0;
@@ -393,7 +454,7 @@ struct LevelK {
         const uint32_t tb = sg->tile_base, ntl = sg->ntiles;
         const U *src = src_keys<U>(p, depth) + begin;
         const uint32_t *isrc = src_idx(p, depth) + begin;
-        const uint8_t *bsrc = p.bins + begin;
+        const uint32_t *bsrc = (const uint32_t *)(p.bins + (size_t)(tb + t) * TILE);
         U *knext = next_keys<U>(p, depth);
         uint32_t *inext = next_idx(p, depth);
         const uint32_t ts = t * TILE;
@@ -405,15 +466,19 @@ struct LevelK {
         uint32_t ix[TI];
         uint32_t pk[TI];   // bin | rank inside the wave's bin << 8
         // keys and the bins the count pass stored for them
+        uint32_t bw[TI / 4];
+#pragma unroll
+        for (int q = 0; q < TI / 4; q++) bw[q] = bsrc[bin_dword(w, q, ln) / 4];
 #pragma unroll
         for (int c = 0; c < TI; c++) {
             const uint32_t e = (uint32_t)w * (TI * WAVE) + c * WAVE + ln;
+            const uint32_t b = (bw[c / 4] >> (8 * (c % 4))) & 0xffu;
             if (FULL) {
                 x[c] = src[ts + e];
-                pk[c] = bsrc[ts + e];
+                pk[c] = b;
             } else {
                 x[c] = e < tl ? src[ts + e] : (U)0;
-                pk[c] = e < tl ? (uint32_t)bsrc[ts + e] : (uint32_t)NB;
+                pk[c] = e < tl ? b : (uint32_t)NB;
             }
         }
         // rank: running per-wave bin counter (one wave owns wcnt[w], its LDS
@@ -520,6 +585,10 @@ struct LevelK {
                 uint32_t sid, t;
                 if (!tile_of(p, g, sid, t)) {
                     if (threadIdx.x == 0) range_fail(p, 6);
+                    continue;
+                }
+                if (g >= p.bins_tiles) {
+                    if (threadIdx.x == 0) range_fail(p, 12);
                     continue;
                 }
                 const Seg *sg = &p.segs[sid];
@@ -1018,15 +1087,23 @@ struct LevelK {
             const U *src = src_keys<U>(p, depth) + begin;
             U k[SPL];
             U mn = key_max<U>(), mx = 0;
+            // len >= SAMPLE: SAMPLE/RUN runs of RUN consecutive keys, run r
+            // starting at r*st + (st - RUN)/2 with st = len / (SAMPLE/RUN), so
+            // that a wave touches one line per run rather than one per key;
+            // sample j = i*WAVE + l is key j % RUN of run j / RUN.  The last
+            // run ends before len: st >= RUN and (SAMPLE/RUN)*st <= len.
+            // Else every key.
+            constexpr uint32_t RUN = 8, NRUN = SAMPLE / RUN;
+            static_assert(SAMPLE >= 8 * RUN && (WAVE % RUN) == 0, "sample runs");
+            const uint32_t st = len / NRUN;
+            const uint32_t base = m == SAMPLE ? (l / RUN) * st + (st - RUN) / 2 + l % RUN : l;
+            const uint32_t step = m == SAMPLE ? (WAVE / RUN) * st : WAVE;
 #pragma unroll
             for (int i = 0; i < (int)SPL; i++) {
                 const uint32_t j = (uint32_t)i * WAVE + l;
                 k[i] = key_max<U>();
                 if (j < m) {
-                    // (2j+1) len / 2m; m is SAMPLE (a power of two) whenever len >= SAMPLE
-                    const uint64_t num = (2ull * j + 1ull) * (uint64_t)len;
-                    const uint64_t pos = m == SAMPLE ? num >> (ilog2(SAMPLE) + 1) : num / (2ull * m);
-                    k[i] = to_ord<U, FL>(src[(uint32_t)pos], flip);
+                    k[i] = to_ord<U, FL>(src[base + (uint32_t)i * step], flip);
                     mn = k[i] < mn ? k[i] : mn;
                     mx = k[i] > mx ? k[i] : mx;
                 }
