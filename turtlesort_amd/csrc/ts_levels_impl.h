@@ -1190,7 +1190,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k
 }
 
 template <int KT, bool IDX, int S>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k_count(Params p)
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kt_bytes(KT) == 1 ? 4 : 6))) k_count(Params p)
 {
     typedef LevelK<KT, IDX, S> K;
     __shared__ typename K::SharedTile sh;
