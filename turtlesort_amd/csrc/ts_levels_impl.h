@@ -553,14 +553,28 @@ struct LevelK {
             __builtin_amdgcn_sched_barrier(0);
         }
         block_sync();
+        // write-out in groups of WU positions per thread: the LDS reads of a
+        // group are issued together and wait once
+        constexpr int WU = 4;
 #pragma nounroll
-        for (uint32_t i = tid; i < tl; i += NT) {
-            {
-                const uint32_t b = stb[i];
-                const uint32_t d = sh.dbase[b < (uint32_t)NB ? b : 0u] + i;
+        for (uint32_t i0 = tid; i0 < tl; i0 += WU * NT) {
+            uint32_t bb[WU], dd[WU];
+#pragma unroll
+            for (int u = 0; u < WU; u++) {
+                const uint32_t i = i0 + (uint32_t)u * NT;
+                bb[u] = (FULL || i < tl) ? (uint32_t)stb[i & (TILE - 1)] : (uint32_t)NB;
+            }
+#pragma unroll
+            for (int u = 0; u < WU; u++) dd[u] = sh.dbase[bb[u] < (uint32_t)NB ? bb[u] : 0u];
+#pragma unroll
+            for (int u = 0; u < WU; u++) {
+                const uint32_t i = i0 + (uint32_t)u * NT;
+                const uint32_t b = bb[u];
+                const uint32_t d = dd[u] + i;
+                const bool real = FULL || i < tl;
                 const bool ok = b < (uint32_t)NB && d - begin < len;   // inside the segment
-                bad |= !ok;
-                if (ok) {
+                bad |= real && !ok;
+                if (real && ok) {
                     U *kd = (b & 1u) ? kfin : knext;
                     kd[d] = stk[i];
                     if (IDX) {
