@@ -809,20 +809,33 @@ struct LevelK {
         U k[SI];
         uint32_t v[SI];
         U mn = key_max<U>(), mx = 0;
-        {
+        if (act) {   // wave-uniform
+            // every load is issued first (positions past the task's end
+            // re-read its last key), then the predicates are applied, so the
+            // loads' latencies overlap
             const U *src = src_keys<U>(p, tr.depth) + tr.begin;
             const uint32_t *isrc = src_idx(p, tr.depth) + tr.begin;
+            const uint32_t last = tr.len ? tr.len - 1u : 0u;
 #pragma unroll
             for (int i = 0; i < SI; i++) {
                 const uint32_t j = j0 + (uint32_t)i * WAVE;
+                const uint32_t jj = j < tr.len ? j : last;
+                k[i] = src[jj];
+                v[i] = IDX ? isrc[jj] : 0u;
+            }
+#pragma unroll
+            for (int i = 0; i < SI; i++) {
+                const bool real = j0 + (uint32_t)i * WAVE < tr.len;
+                k[i] = real ? to_ord<U, FL>(k[i], flip) : key_max<U>();
+                v[i] = real ? v[i] : 0u;
+                mn = real && k[i] < mn ? k[i] : mn;
+                mx = real && k[i] > mx ? k[i] : mx;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < SI; i++) {
                 k[i] = key_max<U>();
                 v[i] = 0u;
-                if (j < tr.len) {
-                    k[i] = to_ord<U, FL>(src[j], flip);
-                    if (IDX) v[i] = isrc[j];
-                    mn = k[i] < mn ? k[i] : mn;
-                    mx = k[i] > mx ? k[i] : mx;
-                }
             }
         }
         mn = wave_min(mn);
@@ -1000,6 +1013,7 @@ struct LevelK {
                 for (int r0 = 0; r0 < SI; r0 += 8) {
                     U kv[8];
                     uint32_t iv[8];
+                    uint32_t ok8 = 0;
 #pragma unroll
                     for (int r = 0; r < 8; r++) {
                         const uint32_t i = tid + (uint32_t)(r0 + r) * NT;
@@ -1007,11 +1021,16 @@ struct LevelK {
                         const TaskRec tr = rec[q];
                         kv[r] = key_max<U>();
                         iv[r] = 0u;
+                        // the loaded value is only used after the loop, so
+                        // the loads of the eight items overlap
                         if (j < tr.len) {
-                            kv[r] = to_ord<U, FL>(src_keys<U>(p, tr.depth)[tr.begin + j], flip);
+                            kv[r] = src_keys<U>(p, tr.depth)[tr.begin + j];
                             if (IDX) iv[r] = src_idx(p, tr.depth)[tr.begin + j];
                         }
+                        ok8 |= (j < tr.len ? 1u : 0u) << r;
                     }
+#pragma unroll
+                    for (int r = 0; r < 8; r++) kv[r] = (ok8 >> r) & 1u ? to_ord<U, FL>(kv[r], flip) : key_max<U>();
                     // pidx(tid + r*NT) = pidx(tid) + r*(NT + NT/SI)
                     U *const mk = sk + pidx<SI>(tid) + (uint32_t)r0 * (NT + NT / SI);
                     uint32_t *const mi = si + pidx<SI>(tid) + (uint32_t)r0 * (NT + NT / SI);
@@ -1112,15 +1131,19 @@ struct LevelK {
             const uint32_t st = len / NRUN;
             const uint32_t base = m == SAMPLE ? (l / RUN) * st + (st - RUN) / 2 + l % RUN : l;
             const uint32_t step = m == SAMPLE ? (WAVE / RUN) * st : WAVE;
+            // all loads first (positions past the sample re-read key 0),
+            // then the predicates, so that the loads' latencies overlap
 #pragma unroll
             for (int i = 0; i < (int)SPL; i++) {
                 const uint32_t j = (uint32_t)i * WAVE + l;
-                k[i] = key_max<U>();
-                if (j < m) {
-                    k[i] = to_ord<U, FL>(src[base + (uint32_t)i * step], flip);
-                    mn = k[i] < mn ? k[i] : mn;
-                    mx = k[i] > mx ? k[i] : mx;
-                }
+                k[i] = src[j < m ? base + (uint32_t)i * step : 0u];
+            }
+#pragma unroll
+            for (int i = 0; i < (int)SPL; i++) {
+                const bool real = (uint32_t)i * WAVE + l < m;
+                k[i] = real ? to_ord<U, FL>(k[i], flip) : key_max<U>();
+                mn = real && k[i] < mn ? k[i] : mn;
+                mx = real && k[i] > mx ? k[i] : mx;
             }
             mn = wave_min(mn);
             mx = wave_max(mx);
