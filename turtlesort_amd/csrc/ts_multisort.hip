@@ -27,16 +27,33 @@ __global__ void __launch_bounds__(NT) ms_kernel(const U *__restrict__ in, U *__r
     const size_t base = (size_t)blockIdx.x * arr_size;
     uint32_t n_eff = next_pow2_dev(arr_size);
     if (n_eff < (uint32_t)ITEMS) n_eff = ITEMS;
-    for (uint32_t i = threadIdx.x; i < n_eff; i += NT) {
-        const bool real = i < arr_size;
-        sk[pidx<ITEMS>(i)] = real ? to_ord<U, FL>(in[base + i], flip) : key_max<U>();
-        if (IDX) si[pidx<ITEMS>(i)] = real && iin ? iin[base + i] : i;
+    // n_eff <= NT*ITEMS: ITEMS strided positions per thread; all loads are
+    // issued before any is used, so that their latencies overlap
+    U kv[ITEMS];
+    uint32_t iv[ITEMS];
+#pragma unroll
+    for (int r = 0; r < ITEMS; r++) {
+        const uint32_t i = threadIdx.x + (uint32_t)r * NT;
+        kv[r] = i < arr_size ? in[base + i] : (U)0;
+        iv[r] = IDX && iin && i < arr_size ? iin[base + i] : i;
+    }
+#pragma unroll
+    for (int r = 0; r < ITEMS; r++) {
+        const uint32_t i = threadIdx.x + (uint32_t)r * NT;
+        if (i < n_eff) {
+            sk[pidx<ITEMS>(i)] = i < arr_size ? to_ord<U, FL>(kv[r], flip) : key_max<U>();
+            if (IDX) si[pidx<ITEMS>(i)] = iv[r];
+        }
     }
     block_sync();
     block_sort<U, IDX, NT, ITEMS>(sk, si, n_eff, n_eff);
-    for (uint32_t i = threadIdx.x; i < arr_size; i += NT) {
-        out[base + i] = from_ord<U, FL>(sk[pidx<ITEMS>(i)], flip);
-        if (IDX) iout[base + i] = si[pidx<ITEMS>(i)];
+#pragma unroll
+    for (int r = 0; r < ITEMS; r++) {
+        const uint32_t i = threadIdx.x + (uint32_t)r * NT;
+        if (i < arr_size) {
+            out[base + i] = from_ord<U, FL>(sk[pidx<ITEMS>(i)], flip);
+            if (IDX) iout[base + i] = si[pidx<ITEMS>(i)];
+        }
     }
 }
 
