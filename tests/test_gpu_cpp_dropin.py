@@ -37,3 +37,11 @@ def test_main_multisort(torch_dev):
     r = run("multisort_main", 2)
     assert r.returncode == 0
     assert r.stdout.count("sort success") == 2
+
+
+def test_multisort_with_index(torch_dev):
+    """MultiSort<double, 2038, 32>(num, keys, index): keys and index equal
+    std::stable_sort of the (key, index) pairs of every array"""
+    r = run("multisort_index", 4000)
+    assert r.returncode == 0
+    assert "sort success" in r.stdout
