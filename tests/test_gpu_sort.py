@@ -167,6 +167,35 @@ def test_device_form_repeats_exact(torch_dev, n):
     sorter.close()
 
 
+@pytest.mark.parametrize("batch", [1, 2])
+def test_level_batches_rerun_lds_phase(monkeypatch, torch_dev, batch):
+    """TURTLE_LEVEL_BATCH caps the partition levels per batch: the host reads
+    back after each batch, runs more levels and then the LDS phase again over
+    all tasks.  The result equals torch's stable sort and the unbatched run."""
+    torch = torch_dev
+    from turtlesort_amd import fill_keys, fill_iota
+    n = 6_000_000
+    sorter = TurtleSort(np.int32, n)
+    sorter.set_tuning(splitters=7)                        # 4 levels at 6M keys
+    kin = torch.empty(n, dtype=torch.int32, device="cuda")
+    iin = torch.empty(n, dtype=torch.int32, device="cuda")
+    fill_keys(kin, 321, 33)
+    fill_iota(iin)
+    k1, i1 = torch.empty_like(kin), torch.empty_like(iin)
+    sorter.sort_device(kin, k1, iin, i1)
+    sorter.synchronize()
+    monkeypatch.setenv("TURTLE_LEVEL_BATCH", str(batch))
+    print(f"case batch={batch}", flush=True)
+    k2, i2 = torch.full_like(kin, -1), torch.full_like(iin, -1)
+    sorter.sort_device(kin, k2, iin, i2)
+    st = sorter.synchronize()
+    assert st["error"] == 0 and st["max_depth"] >= 3
+    ref_k, ref_i = torch.sort(kin.to(torch.int64), stable=True)
+    assert torch.equal(k2.to(torch.int64), ref_k) and torch.equal(i2.to(torch.int64), ref_i)
+    assert torch.equal(k1, k2) and torch.equal(i1, i2)
+    sorter.close()
+
+
 @pytest.mark.parametrize("splitters", [3, 7, 15, 31])
 def test_tuning_variants(orc, torch_dev, splitters):
     """Every splitter count and a few LDS cutoffs / tile granularities give

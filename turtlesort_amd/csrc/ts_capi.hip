@@ -178,6 +178,7 @@ struct turtle_sorter {
     turtle_tuning tune_base{};   // built-in default, or what turtle_set_tuning set
     bool tune_fixed = false;     // set by turtle_set_tuning: the tuning table is not consulted
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t lv_ev = nullptr;   // end of a batch of levels (the read-back waits for it on stream2)
     bool timed = false;
     uint64_t last_n = 0;
     // StartSorting / Sync state
@@ -321,6 +322,8 @@ int ensure_readback(turtle_sorter *h)
     if (!h->levels_host)
         HIPCHK(hipHostMalloc((void **)&h->levels_host, sizeof(ts::Level) + sizeof(ts::Ctrl), hipHostMallocDefault),
                "hipHostMalloc");
+    if (!h->lv_ev) HIPCHK(hipEventCreateWithFlags(&h->lv_ev, hipEventDisableTiming), "hipEventCreate");
+    if (!h->stream2) HIPCHK(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking), "hipStreamCreate");
     return TURTLE_OK;
 }
 
@@ -386,29 +389,47 @@ int run_engine(turtle_sorter *h, const void *kin, const uint32_t *iin, void *kou
                "engine launch");
         h->levels_run = 0;
     } else {
+        // The LDS phase is queued right behind the estimated levels, and the
+        // read-back of the level state runs on the second stream as soon as
+        // the levels are done, while the LDS phase runs: the device never
+        // waits for the host, and the call returns while the LDS phase is
+        // still running.  When the read-back shows segments left, more
+        // levels run and then the LDS phase again over the whole task list:
+        // a task's input range is never written by deeper levels, so sorting
+        // it again writes the same output (its bytes and task counts are
+        // counted twice in the statistics).
+        // TURTLE_LEVEL_BATCH caps the levels per batch (tests force the
+        // read-back path with it)
+        const uint32_t batch_cap = (uint32_t)env_u64("TURTLE_LEVEL_BATCH", 0);
+        if (batch_cap && nlev > batch_cap) nlev = batch_cap;
         uint32_t L0 = 0;
         bool first = true;
         for (;;) {
             if (L0 + nlev > (uint32_t)ts::MAX_LEVELS) nlev = ts::MAX_LEVELS - L0;
+            if (!first) HIPCHK(ts::engine_clear_classes(p, st), "engine launch");
             HIPCHK(ts::engine_levels(p, h->kt, idx, S, h->cus, wpc, dev_off, nseg, (uint32_t)total, L0, nlev, first,
                                      false, st, h->timer),
+                   "engine launch");
+            HIPCHK(hipEventRecord(h->lv_ev, st), "hipEventRecord");
+            HIPCHK(ts::engine_levels(p, h->kt, idx, S, h->cus, wpc, dev_off, nseg, (uint32_t)total, L0 + nlev, 0, false,
+                                     true, st, h->timer),
                    "engine launch");
             first = false;
             L0 += nlev;
             // segments created by the last level are the ones still to do
+            // (seg_alloc and the level record do not change in the LDS phase)
             ts::Ctrl *ch = (ts::Ctrl *)(h->levels_host + 1);
-            HIPCHK(hipMemcpyAsync(h->levels_host, h->levels + (L0 - 1), sizeof(ts::Level), hipMemcpyDeviceToHost, st),
+            HIPCHK(hipStreamWaitEvent(h->stream2, h->lv_ev, 0), "hipStreamWaitEvent");
+            HIPCHK(hipMemcpyAsync(h->levels_host, h->levels + (L0 - 1), sizeof(ts::Level), hipMemcpyDeviceToHost,
+                                  h->stream2),
                    "D2H(level)");
-            HIPCHK(hipMemcpyAsync(ch, h->ctrl, sizeof(ts::Ctrl), hipMemcpyDeviceToHost, st), "D2H(ctrl)");
-            HIPCHK(hipStreamSynchronize(st), "hipStreamSynchronize");
+            HIPCHK(hipMemcpyAsync(ch, h->ctrl, sizeof(ts::Ctrl), hipMemcpyDeviceToHost, h->stream2), "D2H(ctrl)");
+            HIPCHK(hipStreamSynchronize(h->stream2), "hipStreamSynchronize");
             const uint32_t left = ch->seg_alloc > h->levels_host->se ? ch->seg_alloc - h->levels_host->se : 0u;
             if (left == 0 || ch->error != 0 || L0 >= (uint32_t)ts::MAX_LEVELS) break;
-            nlev = 4;
+            nlev = batch_cap && batch_cap < 4 ? batch_cap : 4;
         }
         h->levels_run = L0;
-        HIPCHK(ts::engine_levels(p, h->kt, idx, S, h->cus, wpc, dev_off, nseg, (uint32_t)total, L0, 0, false, true, st,
-                                 h->timer),
-               "engine launch");
     }
     if (!capturing) HIPCHK(hipEventRecord(h->ev1, st), "hipEventRecord");
     h->timed = !capturing;
@@ -725,6 +746,7 @@ int turtle_destroy(turtle_handle h)
         if (h->stage_ev[k]) (void)hipEventDestroy(h->stage_ev[k]);
     }
     if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->lv_ev) (void)hipEventDestroy(h->lv_ev);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     if (h->stream2) (void)hipStreamDestroy(h->stream2);

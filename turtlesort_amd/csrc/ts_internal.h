@@ -27,6 +27,9 @@ hipError_t engine_levels(const Params &p, int kt, bool idx, int splitters, int c
                          const uint32_t *dev_off, uint32_t nseg, uint32_t single_len, uint32_t L0, uint32_t nlev,
                          bool first, bool last, hipStream_t st, LaunchTimer *timer = nullptr);
 
+// Clears the LDS-sort size-class counters before the LDS phase runs again.
+hipError_t engine_clear_classes(const Params &p, hipStream_t st);
+
 // MultiSort kernels (ts_multisort.hip): arrays of arr_size keys, in -> out.
 // kt: key type code.  iout != nullptr: a 32-bit payload moves with every
 // key, read from iin, or each key's position in its array when iin is

@@ -11,6 +11,22 @@ __global__ void k_reset(Ctrl *c)
     for (uint32_t i = threadIdx.x; i < sizeof(Ctrl) / 4; i += blockDim.x) w[i] = 0;
 }
 
+// Clears the LDS-sort size-class counters, so that the LDS phase can run
+// again over the whole task list after further partition levels.
+__global__ void k_clear_classes(Ctrl *c)
+{
+    for (uint32_t i = threadIdx.x; i < (uint32_t)LDS_CLASSES; i += blockDim.x) {
+        c->cls_cnt[i] = 0;
+        c->cls_cur[i] = 0;
+    }
+}
+
+hipError_t engine_clear_classes(const Params &p, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_clear_classes, dim3(1), dim3(64), 0, st, p.ctrl);
+    return hipGetLastError();
+}
+
 // Roots of a batch of segments [off[s], off[s+1]) (off == nullptr: the
 // single segment [0, single_len)): large ones become level-0 segments,
 // small ones LDS tasks.  Appends are aggregated per wave.
