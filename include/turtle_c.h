@@ -139,7 +139,8 @@ int turtle_multisort_index(turtle_handle h, int dtype, int64_t arr_size, int64_t
  * is not modified; keys_out may equal keys_in.  stream is a hipStream_t;
  * NULL is the null (default) stream, so the work is ordered after whatever
  * produced the inputs there.  Returns once the work is queued; the host
- * waits for a small read-back between batches of levels. */
+ * waits for a small read-back after the partition levels (the final LDS
+ * sort may still be running when it returns). */
 int turtle_sort_device(turtle_handle h, const void *keys_in, const int32_t *idx_in,
                        void *keys_out, int32_t *idx_out, int64_t n, void *stream);
 
