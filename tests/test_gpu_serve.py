@@ -86,3 +86,29 @@ def test_server_admission(torch_dev):
     assert np.array_equal(k, np.arange(1000, dtype=np.int32))
     assert server.stats()["rejected"] == 1
     server.close()
+
+
+def test_server_float_requests(orc, torch_dev):
+    """float and double requests (every bit pattern: NaNs, infinities, both
+    zeros) through the request loop, mixed with int32 ones: keys equal the
+    oracle's bit for bit, the index is the stable order of the oracle's map"""
+    sizes = wl.loguniform_lengths(77, 18, 1, 200_000)
+    reqs = []
+    for i, n in enumerate(sizes):
+        dt = (np.float32, np.float64, np.int32)[i % 3]
+        keys = wl.gen_keys(dt, int(n), 9000 + i, 0)
+        idx = np.arange(n, dtype=np.int32) if i % 2 else None
+        reqs.append((keys, idx))
+    server = SortServer(pool_bytes=4 << 30, slot_keys=1 << 20, slots_per_dtype=2,
+                        dtypes=[np.float32, np.float64, np.int32])
+    outs = run_mixed(server, reqs, threads=3)
+    assert server.stats()["failed"] == 0
+    server.close()
+    for i, (keys, idx) in enumerate(reqs):
+        print(f"case request={i} dtype={keys.dtype} n={keys.size}", flush=True)
+        k, x = outs[i]
+        ok, _ = orc.turtle_sort(keys)
+        ui = np.dtype(f"u{keys.dtype.itemsize}")
+        np.testing.assert_array_equal(k.view(ui), ok.view(ui))
+        if idx is not None:
+            np.testing.assert_array_equal(x, np.argsort(orc.order(keys), kind="stable"))
