@@ -798,9 +798,12 @@ struct LevelK {
                                       uint32_t nk, uint32_t tid, RadixShared &rs)
     {
         const U flip = (U)p.flip;
-        const uint64_t mkw = (1ull << 32) / kw + 1ull;
+        // floor(w / kw) = (w * M16[kw]) >> 16, M16[kw] = 2^16/kw + 1: exact
+        // for w < 8 and kw <= 8 (a table instead of a 64-bit division)
+        constexpr uint32_t M16[9] = {0u, 65537u, 32769u, 21846u, 16385u, 13108u, 10923u, 9363u, 8193u};
+        static_assert(NW <= 8, "one workgroup step holds at most 8 waves");
         const uint32_t w = __builtin_amdgcn_readfirstlane(tid / WAVE), l = tid % WAVE;
-        const uint32_t q = (uint32_t)((w * mkw) >> 32);   // task of this wave
+        const uint32_t q = (w * M16[kw < 9u ? kw : 8u]) >> 16;   // task of this wave
         const bool act = q < nk;                          // waves past the step's tasks idle
         const uint32_t j0 = (w - q * kw) * WS + l;        // task position of item i: j0 + i*WAVE
         const TaskRec tr = act ? rec[q] : TaskRec{0, 0, 0};
@@ -860,7 +863,23 @@ struct LevelK {
         uint32_t npass = (rs.maxbits + RW - 1) / RW;
         npass = npass < MAXPASS ? npass : MAXPASS;
         const U tmin = act ? rs.tmin[q] : (U)0;
-        const uint32_t E = nk * kw * NBIN;                // scan entries of the step
+        // scan entries per thread and their (pair, wave) nibbles, the same
+        // for every thread: entry k2 is pair k2 / kw, wave k2 % kw
+        static_assert(NBIN == (uint32_t)NT && EPT <= 8 && NW <= 15, "scan ownership");
+        const uint32_t nent = nk * kw;
+        uint32_t ej = 0, ew = 0;
+        {
+            uint32_t jj = 0, ww = 0;
+#pragma unroll
+            for (int k2 = 0; k2 < (int)EPT; k2++) {
+                ej |= jj << (4 * k2);
+                ew |= ww << (4 * k2);
+                if (++ww == kw) {
+                    ww = 0;
+                    jj++;
+                }
+            }
+        }
         for (uint32_t pass = 0; pass < npass; pass++) {
             const uint32_t sh = pass * RW;
             if (pass > 0 && act) {
@@ -888,26 +907,31 @@ struct LevelK {
             }
             block_sync();
             // exclusive scan over (task, digit, wave of the task): the first
-            // slot of every (wave, digit) run of the step
+            // slot of every (wave, digit) run of the step.  Thread tid owns
+            // the nk (task, digit) pairs tid*nk .. tid*nk+nk-1 (NBIN == NT),
+            // i.e. nk*kw <= EPT entries; entry k2 is pair k2/kw, wave k2%kw
+            // of the task (from the nibbles ej, ew)
             {
-                const uint32_t e0 = tid * EPT;
                 uint32_t cv[EPT], loc = 0;
 #pragma unroll
                 for (int k2 = 0; k2 < (int)EPT; k2++) {
-                    const uint32_t e = e0 + k2;
-                    const uint32_t td = (uint32_t)((e * mkw) >> 32);   // task*NBIN + digit
-                    const uint32_t ww = (td >> RW) * kw + (e - td * kw);
-                    cv[k2] = e < E ? (uint32_t)rs.cnt[ww < NW ? ww : 0][td & (NBIN - 1)] : 0u;
+                    cv[k2] = 0u;
+                    if ((uint32_t)k2 < nent) {   // uniform
+                        const uint32_t td = tid * nk + ((ej >> (4 * k2)) & 15u);   // task*NBIN + digit
+                        const uint32_t row = (td >> RW) * kw + ((ew >> (4 * k2)) & 15u);
+                        cv[k2] = rs.cnt[row][td & (NBIN - 1)];
+                    }
                     loc += cv[k2];
                 }
                 uint32_t tot;
                 uint32_t run = block_excl_scan<NT>(loc, rs.red, &tot);
 #pragma unroll
                 for (int k2 = 0; k2 < (int)EPT; k2++) {
-                    const uint32_t e = e0 + k2;
-                    const uint32_t td = (uint32_t)((e * mkw) >> 32);
-                    const uint32_t ww = (td >> RW) * kw + (e - td * kw);
-                    if (e < E) rs.cnt[ww][td & (NBIN - 1)] = (uint16_t)run;
+                    if ((uint32_t)k2 < nent) {
+                        const uint32_t td = tid * nk + ((ej >> (4 * k2)) & 15u);
+                        const uint32_t row = (td >> RW) * kw + ((ew >> (4 * k2)) & 15u);
+                        rs.cnt[row][td & (NBIN - 1)] = (uint16_t)run;
+                    }
                     run += cv[k2];
                 }
             }
@@ -991,11 +1015,13 @@ struct LevelK {
                 // write-out: task t's sorted keys are at slots [t*region, t*region + len)
                 uint32_t tid2 = threadIdx.x;
                 asm volatile("" : "+v"(tid2));
-                const uint64_t mkw = (1ull << 32) / kw + 1ull;   // floor(x / kw) = (x * mkw) >> 32
+                // floor(x / kw) = (x * M16[kw]) >> 16 for x < 8 (slot wave x)
+                constexpr uint32_t M16[9] = {0u, 65537u, 32769u, 21846u, 16385u, 13108u, 10923u, 9363u, 8193u};
+                const uint32_t mkw = M16[kw < 9u ? kw : 8u];
 #pragma unroll
                 for (int r = 0; r < SI; r++) {
                     const uint32_t s = tid2 + (uint32_t)r * NT;
-                    const uint32_t t = (uint32_t)(((s / WS) * mkw) >> 32);
+                    const uint32_t t = ((s / WS) * mkw) >> 16;
                     const uint32_t j = s - t * region;
                     if (t < nk) {
                         const TaskRec tr = rec[t];
