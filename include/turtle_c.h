@@ -203,6 +203,10 @@ typedef struct turtle_profile {
  * those events, returns the sums since the previous call (or since profiling
  * was switched on) and starts new sums. */
 int turtle_set_profiling(turtle_handle h, int on);
+/* As turtle_set_profiling(h, 1), but only launches of the kernel kinds in
+ * kind_mask (bit 1 << turtle_kernel_kind) get an event pair, so that timing
+ * one kernel disturbs the rest of the run as little as possible; 0 = off. */
+int turtle_set_profiling_kinds(turtle_handle h, uint32_t kind_mask);
 int turtle_get_profile(turtle_handle h, turtle_profile *out);
 
 /* Tuning.  Unless turtle_set_tuning fixed it, every run picks its tuning

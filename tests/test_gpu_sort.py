@@ -213,3 +213,34 @@ def test_tuning_variants(orc, torch_dev, splitters):
             sorter.sync()
             check_against_oracle(orc, keys, k, idx)
     sorter.close()
+
+
+def test_launch_timing(torch_dev):
+    """per-kernel launch timing: every kind when on, only the selected kinds
+    with a mask; the sums match the launches of the runs"""
+    torch = torch_dev
+    from turtlesort_amd import fill_keys, fill_iota
+    n = 4_000_000
+    sorter = TurtleSort(np.int32, n)
+    kin = torch.empty(n, dtype=torch.int32, device="cuda")
+    iin = torch.empty(n, dtype=torch.int32, device="cuda")
+    fill_keys(kin, 17, 33)
+    fill_iota(iin)
+    kout, iout = torch.empty_like(kin), torch.empty_like(iin)
+    sorter.set_profiling(True)
+    for _ in range(2):
+        sorter.sort_device(kin, kout, iin, iout)
+    st = sorter.synchronize()
+    prof = sorter.get_profile()
+    per_run = prof["scatter"]["launches"] // 2              # levels launched per run
+    assert per_run >= st["max_depth"] >= 1 and prof["scatter"]["launches"] == 2 * per_run
+    assert prof["count"]["launches"] == 2 * per_run and prof["pivot"]["launches"] == 2 * per_run
+    assert prof["lds"]["launches"] == 4 and all(prof[k]["ms"] > 0 for k in ("scatter", "count", "lds"))
+    sorter.set_profiling(True, kinds=["scatter"])
+    sorter.sort_device(kin, kout, iin, iout)
+    sorter.synchronize()
+    prof = sorter.get_profile()
+    assert prof["scatter"]["launches"] == per_run and prof["scatter"]["ms"] > 0
+    assert all(v["launches"] == 0 for k, v in prof.items() if k != "scatter")
+    sorter.set_profiling(False)
+    sorter.close()

@@ -125,6 +125,7 @@ SIGNATURES = {
     "turtle_graph_launch": (_i32, [_vp, _vp]),
     "turtle_graph_destroy": (_i32, [_vp]),
     "turtle_set_profiling": (_i32, [_vp, _i32]),
+    "turtle_set_profiling_kinds": (_i32, [_vp, ctypes.c_uint32]),
     "turtle_get_profile": (_i32, [_vp, ctypes.POINTER(Profile)]),
     "turtle_get_tuning": (_i32, [_vp, ctypes.POINTER(Tuning)]),
     "turtle_set_tuning": (_i32, [_vp, ctypes.POINTER(Tuning)]),

@@ -192,8 +192,16 @@ class TurtleSort:
         return st.as_dict()
 
     # ---- per-kernel launch timing ---------------------------------------------------
-    def set_profiling(self, on=True):
-        _lib.check(self._lib.turtle_set_profiling(self._h, int(bool(on))))
+    def set_profiling(self, on=True, kinds=None):
+        """per-kernel launch timing on/off; kinds: names from
+        _lib.KERNEL_KINDS to time only those launches"""
+        if on and kinds is not None:
+            mask = 0
+            for k in kinds:
+                mask |= 1 << _lib.KERNEL_KINDS.index(k)
+            _lib.check(self._lib.turtle_set_profiling_kinds(self._h, mask))
+        else:
+            _lib.check(self._lib.turtle_set_profiling(self._h, int(bool(on))))
 
     def get_profile(self):
         """{kernel kind: {"ms": summed device ms, "launches": n}} since the last call"""

@@ -100,6 +100,7 @@ struct EventTimer : ts::LaunchTimer {
     std::vector<Rec> recs;
     size_t used = 0;
     hipError_t err = hipSuccess;
+    uint32_t mask = ~0u;   // kernel kinds that are timed (bit per kind)
 
     hipEvent_t take()
     {
@@ -116,7 +117,7 @@ struct EventTimer : ts::LaunchTimer {
     }
     void begin(int kind, hipStream_t st) override
     {
-        if (err != hipSuccess) return;
+        if (err != hipSuccess || !((mask >> kind) & 1u)) return;
         hipEvent_t a = take(), b = take();
         if (!a || !b) return;
         const hipError_t r = hipEventRecord(a, st);
@@ -128,7 +129,7 @@ struct EventTimer : ts::LaunchTimer {
     }
     void end(int kind, hipStream_t st) override
     {
-        if (err != hipSuccess || recs.empty() || recs.back().kind != kind) return;
+        if (err != hipSuccess || !((mask >> kind) & 1u) || recs.empty() || recs.back().kind != kind) return;
         const hipError_t r = hipEventRecord(recs.back().b, st);
         if (r != hipSuccess) err = r;
     }
@@ -769,6 +770,16 @@ int turtle_set_profiling(turtle_handle h, int on)
         delete h->timer;
         h->timer = nullptr;
     }
+    return TURTLE_OK;
+}
+
+int turtle_set_profiling_kinds(turtle_handle h, uint32_t kind_mask)
+{
+    if (!h) return fail(TURTLE_ERR_INVALID, "NULL handle");
+    if (kind_mask == 0) return turtle_set_profiling(h, 0);
+    int rc = turtle_set_profiling(h, 1);
+    if (rc) return rc;
+    h->timer->mask = kind_mask;
     return TURTLE_OK;
 }
 
