@@ -353,11 +353,38 @@ struct LevelK {
 #pragma unroll
                 for (int c = 0; c < TI; c++) x[c] = to_ord<U, FL>(x[c], flip);
                 classify_n<U, S, LOG, TI>(x, bins, sh.sp, reg, reg3);
+                // An item whose valid lanes all hold one bin (sorted runs,
+                // heavy duplicates) adds its count with one atomic: 64
+                // same-address atomics would be serialised by the LDS.  The
+                // items are tested only when the wave's first or last item
+                // is such a one, so that random keys pay two tests per tile.
+                const uint32_t ebase = (uint32_t)w * (TI * WAVE) + lane();
 #pragma unroll
-                for (int c = 0; c < TI; c++) {
-                    const uint32_t e = (uint32_t)w * (TI * WAVE) + c * WAVE + lane();
-                    if (e < tl) atomicAdd(&sh.wcnt[w][bins[c]], 1u);
-                    else bins[c] = NB;
+                for (int c = 0; c < TI; c++)
+                    if (ebase + (uint32_t)c * WAVE >= tl) bins[c] = NB;
+                auto uniform = [&](int c, uint32_t &b0, uint64_t &vm) {
+                    const bool ok = ebase + (uint32_t)c * WAVE < tl;
+                    b0 = __builtin_amdgcn_readfirstlane(bins[c]);
+                    vm = ballot(ok);
+                    return ballot(ok && bins[c] == b0) == vm;
+                };
+                uint32_t bf, bl;
+                uint64_t mf, ml;
+                if (uniform(0, bf, mf) || uniform(TI - 1, bl, ml)) {   // wave-uniform
+#pragma unroll
+                    for (int c = 0; c < TI; c++) {
+                        uint32_t b0 = 0;
+                        uint64_t vm = 0;
+                        if (uniform(c, b0, vm)) {
+                            if (lane() == 0 && vm) atomicAdd(&sh.wcnt[w][b0], (uint32_t)__popcll(vm));
+                        } else if (ebase + (uint32_t)c * WAVE < tl) {
+                            atomicAdd(&sh.wcnt[w][bins[c]], 1u);
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int c = 0; c < TI; c++)
+                        if (ebase + (uint32_t)c * WAVE < tl) atomicAdd(&sh.wcnt[w][bins[c]], 1u);
                 }
                 // the scatter reads the bins back (items past the tile's end
                 // hold the marker NB)
