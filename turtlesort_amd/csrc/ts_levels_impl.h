@@ -726,7 +726,7 @@ struct LevelK {
     {
         const uint32_t ntask = p.ctrl->lds_alloc < p.ldscap ? p.ctrl->lds_alloc : p.ldscap;
         const uint32_t stride = gridDim.x * blockDim.x;
-        if (threadIdx.x == 0) lds_plan(p.ctrl, CRAD, NW, CAP, coff, cwg);
+        if (threadIdx.x == 0) lds_plan(p.ctrl, CPLAN, NW, CAP, coff, cwg);
         block_sync();
         for (uint32_t k0 = blockIdx.x * blockDim.x; k0 < ntask; k0 += stride) {
             if (threadIdx.x < (uint32_t)LDS_CLASSES) hist[threadIdx.x] = 0;
@@ -759,9 +759,15 @@ struct LevelK {
     static constexpr uint32_t WS = C::WS;
     static constexpr uint32_t CRAD = (uint32_t)C::CRAD, CLAST = (uint32_t)C::CLAST;
 
+    // 8-byte keys sort every task with the merge path: after the partition
+    // their tasks still span ~50 bits for full-range keys, i.e. six 64-bit
+    // radix passes, which cost more than the log2(n/SI) merge rounds (the
+    // batched 20,000-segment int64 run: 12.7 -> 10.3 ms in the LDS sorts)
+    static constexpr bool MERGE_ONLY = (KB == 8);
+    static constexpr uint32_t CPLAN = MERGE_ONLY ? (uint32_t)C::CMAX + 1 : (uint32_t)CRAD;   // plan's first radix class
     static __device__ __forceinline__ uint32_t lds_class(uint32_t len)
     {
-        if (len <= WS / 2) {
+        if (MERGE_ONLY || len <= WS / 2) {
             const uint32_t c = len <= 1 ? 0u : (uint32_t)(32 - __clz(len - 1));
             return c < (uint32_t)C::CMIN ? (uint32_t)C::CMIN : c;
         }
@@ -997,9 +1003,12 @@ struct LevelK {
         const U flip = (U)p.flip;
         U *dst = (U *)p.kout;
         const Ctrl *cc = p.ctrl;
-        if (threadIdx.x == 0) lds_plan(cc, CRAD, NW, CAP, rs.coff, rs.cwg);
+        if (threadIdx.x == 0) lds_plan(cc, CPLAN, NW, CAP, rs.coff, rs.cwg);
         block_sync();
-        const uint32_t c0 = RAD ? CRAD : (uint32_t)C::CMIN, clast = RAD ? CLAST : CRAD - 1;
+        // merge classes [CMIN, CPLAN), radix classes [CRAD, CLAST] (none when
+        // MERGE_ONLY: the merge classes then reach CMAX)
+        const uint32_t c0 = RAD ? (MERGE_ONLY ? (uint32_t)CLAST + 1 : (uint32_t)CRAD) : (uint32_t)C::CMIN,
+                       clast = RAD ? (uint32_t)CLAST : CPLAN - 1;
         const uint32_t vbeg = rs.cwg[c0], vend = rs.cwg[clast + 1];
         uint64_t nbytes = 0, ntasks = 0;
         if (threadIdx.x == 0) acc[0] = acc[1] = 0;
@@ -1397,7 +1406,8 @@ hipError_t run_levels(const Params &p, int cus, int wg_per_cu, const uint32_t *d
         TS_LAUNCH(K_LDSPREP, (k_ldscount<KT, IDX, S>), dim3(g_small), dim3(256), 0, st, p);
         TS_LAUNCH(K_LDSPREP, (k_ldsbucket<KT, IDX, S>), dim3(g_small), dim3(256), 0, st, p);
         TS_LAUNCH(K_LDS, (k_lds<KT, IDX, S, false>), dim3(g_lds), dim3(512), 0, st, p);
-        TS_LAUNCH(K_LDS, (k_lds<KT, IDX, S, true>), dim3(g_lds), dim3(512), 0, st, p);
+        if (!LevelK<KT, IDX, S>::MERGE_ONLY)
+            TS_LAUNCH(K_LDS, (k_lds<KT, IDX, S, true>), dim3(g_lds), dim3(512), 0, st, p);
     }
 #undef TS_LAUNCH
     return hipGetLastError();
