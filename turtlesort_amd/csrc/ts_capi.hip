@@ -346,12 +346,29 @@ turtle_tuning tuning_for(const turtle_sorter *h, bool idx, uint64_t total);
 // the first batch).  A graph capture cannot read back, so it captures a
 // generous fixed number of levels and turtle_synchronize reports ERR_DEPTH
 // if that was not enough.
+// longest root of a run: the segments' longest (h->host_off holds the
+// offsets of a segmented run), else the whole input
+uint64_t longest_root(const turtle_sorter *h, bool segmented, uint64_t total)
+{
+    if (!segmented || h->host_off.size() < 2) return total;
+    uint64_t longest = 0;
+    for (size_t s = 0; s + 1 < h->host_off.size(); s++) {
+        const uint64_t l = (uint64_t)h->host_off[s + 1] - h->host_off[s];
+        if (l > longest) longest = l;
+    }
+    return longest;
+}
+
 int run_engine(turtle_sorter *h, const void *kin, const uint32_t *iin, void *kout, uint32_t *iout, uint64_t total,
                const uint32_t *dev_off, uint32_t nseg, hipStream_t st, bool capturing = false)
 {
     const bool idx = iin != nullptr && iout != nullptr;
-    // (a capture uses the tuning its creator resolved and sized the tables for)
-    if (!capturing) h->tune = tuning_for(h, idx, total);
+    // the longest root decides the level count, and the tuning: the segments
+    // of a batch are partitioned independently, so a batch is tuned like one
+    // sort of its longest segment (a capture uses the tuning its creator
+    // resolved and sized the tables for)
+    const uint64_t longest = longest_root(h, dev_off != nullptr, total);
+    if (!capturing) h->tune = tuning_for(h, idx, longest);
     int rc = ensure_tables(h, total, nseg, 1);
     if (rc) return rc;
     if ((rc = ensure_readback(h))) return rc;
@@ -366,15 +383,6 @@ int run_engine(turtle_sorter *h, const void *kin, const uint32_t *iin, void *kou
     ts::Params p = make_params(h, kin, idx ? iin : nullptr, kout, idx ? iout : nullptr);
     p.n_total = (uint32_t)total;
     const int S = h->tune.splitters, wpc = h->tune.workgroups_per_cu;
-    // the longest root decides the level count
-    uint64_t longest = total;
-    if (dev_off && h->host_off.size() > 1) {
-        longest = 0;
-        for (size_t s = 0; s + 1 < h->host_off.size(); s++) {
-            const uint64_t l = (uint64_t)h->host_off[s + 1] - h->host_off[s];
-            if (l > longest) longest = l;
-        }
-    }
     uint32_t nlev = levels_estimate(longest, (uint64_t)h->tune.lds_cutoff, (uint64_t)S);
     if (!capturing) HIPCHK(hipEventRecord(h->ev0, st), "hipEventRecord");
     if (capturing) {
@@ -1222,7 +1230,7 @@ int turtle_segmented_graph_create(turtle_handle h, const void *keys_in, const in
     for (size_t s = 0; s < off.size(); s++) off[s] = (uint32_t)offsets[s];
     h->host_off = off;   // the level estimate reads the longest segment from here
     // everything the run allocates is allocated before the capture starts
-    h->tune = tuning_for(h, idx_in != nullptr, (uint64_t)total);
+    h->tune = tuning_for(h, idx_in != nullptr, longest_root(h, true, (uint64_t)total));
     int rc = ensure_tables(h, (uint64_t)total, (uint32_t)num_segments, 1);
     if (!rc) rc = ensure_readback(h);
     hipError_t e = hipSuccess;
