@@ -801,6 +801,15 @@ struct LevelK {
         U buf[NW][C::SAMPLE];         // per-wave sample
     };
 
+    template <typename T>
+    static __device__ __forceinline__ T shfl_u(T x, int src)
+    {
+        if constexpr (sizeof(T) == 8)
+            return (T)__shfl((unsigned long long)x, src, WAVE);
+        else
+            return (T)__shfl((uint32_t)x, src, WAVE);
+    }
+
     static __device__ __forceinline__ U wave_min(U x)
     {
 #pragma unroll
@@ -1209,11 +1218,16 @@ struct LevelK {
             }
             mn = wave_min(mn);
             mx = wave_max(mx);
+            // The sample is sorted on the top 2*RW bits of its span only (at
+            // most two passes): the keys at the quantile ranks are then within
+            // 2^-18 of the span of the exact quantiles, and the picked
+            // splitters are put in exact order below.
             const uint64_t span = mx > mn ? (uint64_t)(U)(mx - mn) : 0ull;
             const uint32_t bits = span ? (uint32_t)(64 - __clzll(span)) : 0u;
-            const uint32_t npass = (bits + RW - 1) / RW;
+            const uint32_t low = bits > 2u * RW ? bits - 2u * RW : 0u;   // bits left unsorted
+            const uint32_t npass = (bits - low + RW - 1) / RW;
             for (uint32_t pass = 0; pass < npass; pass++) {
-                const uint32_t sh = pass * RW;
+                const uint32_t sh = low + pass * RW;
                 for (uint32_t d = l; d < NBIN; d += WAVE) cn[d] = 0;
                 wave_sync();
                 uint32_t dr[SPL];
@@ -1258,7 +1272,17 @@ struct LevelK {
                 for (int i = 0; i < (int)SPL; i++) b[(uint32_t)i * WAVE + l] = k[i];
                 wave_sync();
             }
-            if (l < (uint32_t)S) sg->split[l] = (uint64_t)b[((l + 1) * m) / (S + 1)];
+            // splitter l = the key at quantile rank (l+1) m / (S+1), then each
+            // lane's rank among the S picked keys (ties by lane) orders them
+            // exactly (they are already ordered when the sort was exact)
+            const U spv = l < (uint32_t)S ? b[((l + 1) * m) / (S + 1)] : key_max<U>();
+            uint32_t srank = 0;
+#pragma unroll
+            for (int t = 0; t < S; t++) {
+                const U o = shfl_u<U>(spv, t);
+                srank += (o < spv || (o == spv && (uint32_t)t < l)) ? 1u : 0u;
+            }
+            if (l < (uint32_t)S) sg->split[srank] = (uint64_t)spv;
             const uint32_t ntiles = (len + TILE - 1) / TILE;
             uint32_t tb = 0;
             if (l == 0) {
