@@ -1,7 +1,7 @@
 // ts_levels_impl.h -- the sort engine for gfx950: level-synchronous
 // partition passes over a device-resident work list, then LDS sorts.
-// Included once per key width by ts_levels_kb{1,2,4,8}.hip, which
-// instantiate it; ts_levels.hip dispatches.
+// Included once per key type by ts_levels_kb{1,2,4,8}.hip and
+// ts_levels_kf{4,8}.hip, which instantiate it; ts_levels.hip dispatches.
 //
 // The reference recurses with CUDA dynamic parallelism: each wave of
 // quicksort blocks tail-launches the next wave (turtle-sort.cu:35-111,
@@ -13,12 +13,13 @@
 //
 //   k_pivot    opens level L (its segments are ids [sb, se) of the segment
 //              table, created by the previous level); per segment: sample
-//              1024 keys evenly, sort them in LDS, take
+//              1024 keys (runs of 8 evenly spaced), sort them in LDS, take
 //              S splitters at the sample quantiles (S = 3 is the reference's
 //              3-pivot scheme; 7, 15 and 31 cut a segment into more bins per
 //              pass), reserve the segment's tiles;
 //   k_count    per tile: bin counts; every key's bin is also stored (one
-//              byte) so that the scatter does not search the splitters again;
+//              byte, four to a dword in a per-tile slot) so that the scatter
+//              does not search the splitters again;
 //   k_scan1/2  level-wide exclusive scan of the bin-major tile counts: the
 //              destination of every (segment, bin, tile) run;
 //   k_scatter  per tile: stable wave64 ranking into bins, staged in LDS,
@@ -28,7 +29,8 @@
 //   k_children per segment (one wave): every non-empty non-pivot bin becomes
 //              a segment of level L+1 or, at or below the cutoff, an LDS task.
 // After the last level the LDS tasks are grouped by size class
-// (k_ldscount, k_ldsbucket) and two k_lds kernels sort them.
+// (k_ldscount, k_ldsbucket) and the k_lds kernels sort them (merge sort for
+// small tasks and for 8-byte keys, LSD radix for the others).
 //
 // Bins follow the reference's order LL,=pL,L,=p,R,=pR,RR
 // (turtle-sort.cu:755-761) generalised to S splitters: bin 2j+1 holds keys
@@ -1062,11 +1064,11 @@ struct LevelK {
                 asm volatile("" : "+v"(tid2));
                 // floor(x / kw) = (x * M16[kw]) >> 16 for x < 8 (slot wave x)
                 constexpr uint32_t M16[9] = {0u, 65537u, 32769u, 21846u, 16385u, 13108u, 10923u, 9363u, 8193u};
-                const uint32_t mkw = M16[kw < 9u ? kw : 8u];
+                const uint32_t m16 = M16[kw < 9u ? kw : 8u];
 #pragma unroll
                 for (int r = 0; r < SI; r++) {
                     const uint32_t s = tid2 + (uint32_t)r * NT;
-                    const uint32_t t = ((s / WS) * mkw) >> 16;
+                    const uint32_t t = ((s / WS) * m16) >> 16;
                     const uint32_t j = s - t * region;
                     if (t < nk) {
                         const TaskRec tr = rec[t];
@@ -1146,11 +1148,11 @@ struct LevelK {
 
     // ---------------------------------------------------------------- pivot
     // One wave per segment (8 per workgroup step): the wave samples 1024 keys
-    // at evenly spaced positions into registers (16 per lane), sorts them by
-    // stable LSD radix passes of RW-bit digits over the sample's key range
-    // (ranks from wave64 match masks and per-wave digit counters, no block
-    // barrier), takes S splitters at the sample quantiles and reserves the
-    // segment's tiles.
+    // (128 evenly spaced runs of 8) into registers (16 per lane), sorts them
+    // by stable LSD radix passes of RW-bit digits over the top 2*RW bits of
+    // the sample's key range (ranks from wave64 match masks and per-wave
+    // digit counters, no block barrier), takes S splitters at the sample
+    // quantiles, orders them exactly and reserves the segment's tiles.
     static constexpr uint32_t SPL = C::SAMPLE / WAVE;   // sample keys per lane
 
     static __device__ void pivot(const Params &p, uint32_t L, PivotShared &ps)
