@@ -253,10 +253,12 @@ __device__ __forceinline__ void block_sort(U *sk, uint32_t *si, uint32_t n_eff, 
         block_sync();
         if (active) {
             // The thread's ITEMS outputs are CH independent merge-path
-            // chains of ITEMS/CH outputs each: their binary searches and
-            // serial merges have no dependency on each other, so their LDS
-            // latencies overlap.
-            constexpr int CH = ITEMS >= 8 ? 2 : 1;
+            // chains of ITEMS/CH outputs each.  More chains overlap the LDS
+            // latencies of their searches and merges but each pays its own
+            // binary search; on gfx950 one chain measured fastest (MultiSort
+            // of 1024-key arrays 0.46 -> 0.33 ms per 64M keys against two
+            // chains, four were slower still).
+            constexpr int CH = 1;
             constexpr int PER = ITEMS / CH;
             uint32_t ai[CH], bi[CH], aend[CH], bend[CH];
             U ka[CH], kb[CH];
