@@ -132,3 +132,26 @@ def test_lds_size_classes(orc, torch_dev, dtype):
         assert st["error"] == 0 and st["segments"] == 0     # every segment went straight to the LDS sort
         check_segments(orc, keys, off, d2h(torch, kout), d2h(torch, iout))
     sorter.close()
+
+
+def test_segmented_level_batches(monkeypatch, orc, torch_dev):
+    """a segmented batch with the partition levels capped to one per
+    read-back (TURTLE_LEVEL_BATCH=1): the LDS phase runs again after every
+    extra level; the result equals the oracle's"""
+    torch = torch_dev
+    monkeypatch.setenv("TURTLE_LEVEL_BATCH", "1")
+    lens = wl.loguniform_lengths(53, 60, 1, 400_000)
+    off = wl.segment_offsets(lens)
+    n = int(off[-1])
+    keys = wl.gen_keys(np.int64, n, 61, wl.SHIFT_63BIT)
+    sorter = TurtleSort(np.int64, n)
+    sorter.set_tuning(splitters=7, lds_cutoff=1024)         # several levels
+    kin = h2d(torch, keys)
+    iin = torch.arange(n, dtype=torch.int32, device="cuda")
+    kout = torch.full_like(kin, -1)
+    iout = torch.full_like(iin, -1)
+    sorter.segmented_sort_device(kin, kout, off, iin, iout)
+    st = sorter.synchronize()
+    assert st["error"] == 0 and st["max_depth"] >= 3
+    check_segments(orc, keys, off, d2h(torch, kout), d2h(torch, iout))
+    sorter.close()
