@@ -69,7 +69,8 @@ typedef struct turtle_stats {
 typedef struct turtle_tuning {
     int32_t splitters;         /* pivots per partition pass: 3 (the reference's
                                   3-pivot partition), 7, 15 or 31 */
-    int32_t lds_cutoff;        /* segments at or below this length are sorted in LDS */
+    int32_t lds_cutoff;        /* segments at or below this length are sorted in LDS;
+                                  1024 .. 8192 (4096 for 8-byte keys) */
     int32_t tiles_per_task;    /* tiles a workgroup takes per step of the count and
                                   scatter loops */
     int32_t workgroups_per_cu; /* grid of the tile kernels = CUs x this (0 = occupancy

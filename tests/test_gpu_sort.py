@@ -201,10 +201,10 @@ def test_tuning_variants(orc, torch_dev, splitters):
     """Every splitter count and a few LDS cutoffs / tile granularities give
     the oracle's keys and the stable index order."""
     sorter = TurtleSort(np.int32, 600_000)
-    for cutoff, tiles in [(8192, 1), (1000, 2), (16, 4)]:
+    for cutoff, tiles in [(8192, 1), (2048, 2), (1024, 4)]:
         sorter.set_tuning(splitters=splitters, lds_cutoff=cutoff, tiles_per_task=tiles)
         for dist in ["full", "dup15", "asc", "few16"]:
-            n = 600_000 if cutoff > 16 else 70_000
+            n = 600_000
             print(f"case S={splitters} cutoff={cutoff} tiles={tiles} dist={dist} n={n}", flush=True)
             keys = make_keys(np.int32, n, dist, 11 + cutoff)
             k = keys.copy()

@@ -498,10 +498,16 @@ int error_status(uint32_t err)
     return fail(TURTLE_ERR_DEVICE, "device error");
 }
 
+constexpr uint32_t TUNE_MIN_CUTOFF = 1024;
+
 bool valid_tuning(const turtle_tuning &t, int kb)
 {
     if (!ts::engine_supports(t.splitters)) return false;
-    if (t.lds_cutoff < 16 || (uint32_t)t.lds_cutoff > ts::engine_cap(kb)) return false;
+    // every segment of a level is longer than the cutoff, and each of its
+    // tiles owns a TILE-byte bins slot, so the bins buffer takes up to
+    // n * (1 + TILE / cutoff) bytes: the cutoff's lower bound keeps that
+    // below 9 bytes per key
+    if (t.lds_cutoff < (int32_t)TUNE_MIN_CUTOFF || (uint32_t)t.lds_cutoff > ts::engine_cap(kb)) return false;
     if (t.tiles_per_task < 1 || t.tiles_per_task > 1024) return false;
     if (t.workgroups_per_cu < 0 || t.workgroups_per_cu > 64) return false;
     return true;
