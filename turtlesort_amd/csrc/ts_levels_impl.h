@@ -649,47 +649,70 @@ struct LevelK {
     }
 
     // ---------------------------------------------------------------- children
-    // one wave per segment of level L; lane j handles the non-pivot bin 2j
-    static __device__ void children(const Params &p, uint32_t L)
+    // one wave per segment of level L (CW waves per workgroup step); lane j
+    // handles the non-pivot bin 2j.  The workgroup reserves the step's child
+    // segments and LDS tasks with one atomic per counter (a level of 1024
+    // segments otherwise queues 2048 atomics on two words).
+    static constexpr uint32_t CW = 4;   // waves per children workgroup (256 threads)
+    struct ChildShared {
+        uint32_t nb[CW], ns[CW];
+        uint32_t sbase, lbase;
+    };
+
+    static __device__ void children(const Params &p, uint32_t L, ChildShared &cs)
     {
         const Level lv = p.levels[L];
         // the tiles of this level are done with: the next level's pivot
         // kernel reserves its tiles from zero
         if (blockIdx.x == 0 && threadIdx.x == 0) p.ctrl->tile_alloc = 0;
-        const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
-        const uint32_t nwv = gridDim.x * blockDim.x / WAVE;
-        for (uint32_t sid = lv.sb + wid; sid < lv.se; sid += nwv) {
-            const Seg *sg = &p.segs[sid];
-            const uint32_t nt = sg->ntiles;
-            if (nt == 0) continue;   // overflowed segment
-            const uint32_t begin = sg->begin, len = sg->len, depth = sg->depth;
-            const size_t e0 = (size_t)sg->tile_base * NB, e1 = e0 + (size_t)NB * nt;
-            const uint32_t p0 = p.tpre[e0];
-            const uint32_t sum = p.tpre[e1 - 1] + p.tcnt[e1 - 1] - p0;
-            if (sum != len) {
-                if (lane() == 0) atomic_or(&p.ctrl->error, ERR_BAD);
-                continue;
-            }
+        const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+        for (uint32_t s0 = lv.sb + blockIdx.x * CW; s0 < lv.se; s0 += gridDim.x * CW) {   // block-uniform
+            const uint32_t sid = s0 + w;
             const uint32_t j = lane();
-            uint32_t cl = 0, cb = 0;
-            if (j <= (uint32_t)S) {
-                const uint32_t b = 2 * j;
-                const uint32_t st = p.tpre[e0 + (size_t)b * nt] - p0;
-                const uint32_t en = (b + 1 < (uint32_t)NB) ? p.tpre[e0 + (size_t)(b + 1) * nt] - p0 : len;
-                cl = en - st;
-                cb = begin + st;
+            uint32_t cl = 0, cb = 0, depth = 0;
+            if (sid < lv.se) {   // wave-uniform
+                const Seg *sg = &p.segs[sid];
+                const uint32_t nt = sg->ntiles;
+                if (nt != 0) {   // else an overflowed segment
+                    const uint32_t begin = sg->begin, len = sg->len;
+                    depth = sg->depth;
+                    const size_t e0 = (size_t)sg->tile_base * NB, e1 = e0 + (size_t)NB * nt;
+                    const uint32_t p0 = p.tpre[e0];
+                    const uint32_t sum = p.tpre[e1 - 1] + p.tcnt[e1 - 1] - p0;
+                    if (sum != len) {
+                        if (j == 0) atomic_or(&p.ctrl->error, ERR_BAD);
+                    } else if (j <= (uint32_t)S) {
+                        const uint32_t b = 2 * j;
+                        const uint32_t st = p.tpre[e0 + (size_t)b * nt] - p0;
+                        const uint32_t en = (b + 1 < (uint32_t)NB) ? p.tpre[e0 + (size_t)(b + 1) * nt] - p0 : len;
+                        cl = en - st;
+                        cb = begin + st;
+                    }
+                }
             }
             const bool big = cl > p.cutoff;
             const bool small = cl > 0 && !big;
             const uint64_t mb = ballot(big), ms = ballot(small);
-            uint32_t sbase = 0, lbase = 0;
-            if (lane() == 0) {
-                const uint32_t nb = (uint32_t)__popcll(mb), ns = (uint32_t)__popcll(ms);
-                if (nb) sbase = atomic_add(&p.ctrl->seg_alloc, nb);
-                if (ns) lbase = lds_reserve(p, ns);
+            if (j == 0) {
+                cs.nb[w] = (uint32_t)__popcll(mb);
+                cs.ns[w] = (uint32_t)__popcll(ms);
             }
-            sbase = __shfl(sbase, 0, WAVE);
-            lbase = __shfl(lbase, 0, WAVE);
+            block_sync();
+            if (threadIdx.x == 0) {
+                uint32_t tb = 0, tsm = 0;
+                for (uint32_t x = 0; x < CW; x++) {
+                    tb += cs.nb[x];
+                    tsm += cs.ns[x];
+                }
+                cs.sbase = tb ? atomic_add(&p.ctrl->seg_alloc, tb) : 0u;
+                cs.lbase = tsm ? lds_reserve(p, tsm) : 0u;
+            }
+            block_sync();
+            uint32_t sbase = cs.sbase, lbase = cs.lbase;
+            for (uint32_t x = 0; x < w; x++) {
+                sbase += cs.nb[x];
+                lbase += lbase != ~0u ? cs.ns[x] : 0u;
+            }
             if (big) {
                 const uint32_t cid = sbase + mask_rank(mb);
                 if (cid >= p.segcap) {
@@ -706,6 +729,7 @@ struct LevelK {
                 tk->len = cl;
                 tk->depth = depth + 1;
             }
+            block_sync();   // the next step rewrites cs
         }
     }
 
@@ -1175,9 +1199,10 @@ struct LevelK {
         U *const b = ps.buf[w];
         uint16_t *const cn = ps.cnt[w];
         uint32_t done = 0;
-        for (uint32_t s0 = lv.sb + blockIdx.x * NW; s0 < lv.se; s0 += gridDim.x * NW) {
-            const uint32_t sid = s0 + w;
-            if (sid >= lv.se) continue;   // wave-uniform
+        // segment sb + w*grid + block (+ k*grid*NW): a level with fewer
+        // segments than waves in the grid spreads over every workgroup, one
+        // wave per SIMD, instead of filling the first workgroups' 8 waves
+        for (uint32_t sid = lv.sb + w * gridDim.x + blockIdx.x; sid < lv.se; sid += gridDim.x * NW) {   // wave-uniform
             done++;
             Seg *sg = &p.segs[sid];
             const uint32_t begin = sg->begin, len = sg->len, depth = sg->depth;
@@ -1350,7 +1375,10 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k
 template <int KT, bool IDX, int S>
 __global__ void __launch_bounds__(256) k_children(Params p, uint32_t L)
 {
-    LevelK<KT, IDX, S>::children(p, L);
+    typedef LevelK<KT, IDX, S> K;
+    static_assert(K::CW * WAVE == 256, "k_children runs CW waves per workgroup");
+    __shared__ typename K::ChildShared cs;
+    K::children(p, L, cs);
 }
 
 template <int KT, bool IDX, int S>
