@@ -52,6 +52,9 @@ def parse():
     ap.add_argument("--cutoff", type=int, default=0, help="override the LDS-sort cutoff")
     ap.add_argument("--tiles", type=int, default=0, help="override the tiles per workgroup step")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-segments comparison")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="profiling mode: only the batched workload's graph replays (no plain-sort loop), "
+                         "one JSON line of their figures instead of the bench line")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving run")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-rate", type=float, default=2000.0)
@@ -143,7 +146,7 @@ def check_sorted(torch, kin, kout, iin, iout):
     return ok
 
 
-def bench_batched(torch, np, wl, TurtleSort, fill_keys, reps):
+def bench_batched(torch, np, wl, TurtleSort, fill_keys, reps, graph_only=False):
     """20,000 uneven segments (log-uniform [16, 262144]) of int64 keys + int32
     index, device-resident: one captured hipGraph replay of the segmented
     engine run vs a loop of plain device sorts (turtle_sort_device, the
@@ -176,6 +179,13 @@ def bench_batched(torch, np, wl, TurtleSort, fill_keys, reps):
     ok &= bool(torch.equal(kin[iout.long()], kout)) and bool(torch.equal(seg_id[iout.long()], seg_id))
     del seg_id, same
     g.close()
+    alg = int(st["bytes_count"] + st["bytes_scatter"] + st["bytes_lds"])
+    if graph_only:
+        sorter.close()
+        return {"mode": "batched graph replays only (profiling)", "keys": n, "segments": len(lens), "reps": reps,
+                "batched_graph_Gkeys_s": round(n / t_graph / 1e9, 4), "batched_graph_ms": round(t_graph * 1e3, 3),
+                "alg_bytes": alg, "hbm_frac": round(alg / t_graph / 1e9 / HBM_PEAK_GBS, 4),
+                "max_depth": st["max_depth"], "valid": ok}
     # the same segments through the segmented call without a graph
     sorter.segmented_sort_device(kin, kout, off, iin, iout)
     sorter.synchronize()
@@ -205,9 +215,8 @@ def bench_batched(torch, np, wl, TurtleSort, fill_keys, reps):
         "batched_call_Gkeys_s": round(n / t_seg / 1e9, 4),
         "plain_loop_Gkeys_s": round(n / t_loop / 1e9, 4), "plain_loop_ms": round(t_loop * 1e3, 1),
         "batched_vs_plain": round(t_loop / t_graph, 2),
-        "alg_bytes": int(st["bytes_count"] + st["bytes_scatter"] + st["bytes_lds"]),
-        "hbm_frac": round((st["bytes_count"] + st["bytes_scatter"] + st["bytes_lds"]) / t_graph / 1e9
-                          / HBM_PEAK_GBS, 4),
+        "alg_bytes": alg,
+        "hbm_frac": round(alg / t_graph / 1e9 / HBM_PEAK_GBS, 4),
         "valid": ok,
     }
 
@@ -317,6 +326,9 @@ def main():
         return
 
     torch.cuda.set_device(0)
+    if args.batched_only:
+        print(json.dumps(bench_batched(torch, np, wl, TurtleSort, fill_keys, reps=args.steps, graph_only=True)))
+        return
     n = wl.C2_N
     kb, ib = 4, 4
     sorter = TurtleSort(np.int32, n)
