@@ -231,20 +231,31 @@ __global__ void k_roots(Params p, const uint32_t *off, uint32_t nseg, uint32_t s
 // (class c < crad: cap / 2^c tasks per step; c >= crad: tasks of
 // c - crad + 1 whole waves, nw / (c - crad + 1) per step); wg[LDS_CLASSES]
 // is the total.  Every workgroup that needs it computes it (20 entries).
+// Mixed steps: a step of two 3-wave tasks leaves 2 of 8 waves idle, so the
+// first *mix steps of the 3-wave class also take one 2-wave task each (the
+// first *mix tasks of that class, whose own steps start after them).
 __device__ __forceinline__ void lds_plan(const Ctrl *c, uint32_t crad, uint32_t nw, uint32_t cap, uint32_t *off,
-                                         uint32_t *wg)
+                                         uint32_t *wg, uint32_t *mix)
 {
+    const uint32_t k2 = crad + 1, k3 = crad + 2;   // the 2-wave and 3-wave radix classes
+    uint32_t m = 0;
+    if (nw == 8 && k3 < (uint32_t)LDS_CLASSES) {
+        const uint32_t n2 = c->cls_cnt[k2], n3 = c->cls_cnt[k3];
+        m = (n3 + 1) / 2 < n2 ? (n3 + 1) / 2 : n2;
+    }
     uint32_t o = 0, g = 0;
     for (uint32_t k = 0; k < (uint32_t)LDS_CLASSES; k++) {
         off[k] = o;
         wg[k] = g;
         const uint32_t n = c->cls_cnt[k];
         o += n;
+        const uint32_t ns = k == k2 ? n - m : n;   // tasks in the class's own steps
         uint32_t P = k < crad ? cap >> k : (k - crad + 1 <= nw ? nw / (k - crad + 1) : 1u);
         if (P == 0) P = 1;
-        if (n) g += (n + P - 1) / P;
+        if (ns) g += (ns + P - 1) / P;
     }
     wg[LDS_CLASSES] = g;
+    if (mix) *mix = m;
 }
 
 template <int KT, bool IDX, int S>
@@ -752,7 +763,7 @@ struct LevelK {
     {
         const uint32_t ntask = p.ctrl->lds_alloc < p.ldscap ? p.ctrl->lds_alloc : p.ldscap;
         const uint32_t stride = gridDim.x * blockDim.x;
-        if (threadIdx.x == 0) lds_plan(p.ctrl, CPLAN, NW, CAP, coff, cwg);
+        if (threadIdx.x == 0) lds_plan(p.ctrl, CPLAN, NW, CAP, coff, cwg, nullptr);
         block_sync();
         for (uint32_t k0 = blockIdx.x * blockDim.x; k0 < ntask; k0 += stride) {
             if (threadIdx.x < (uint32_t)LDS_CLASSES) hist[threadIdx.x] = 0;
@@ -815,6 +826,10 @@ struct LevelK {
     struct RadixShared {
         uint32_t coff[LDS_CLASSES];       // LDS-sort plan (lds_plan)
         uint32_t cwg[LDS_CLASSES + 1];
+        uint32_t mix;                     // mixed 3+3+2-wave steps (lds_plan)
+        // the step's tasks: task of wave w (NW when idle), first wave and
+        // waves of task t (tasks hold consecutive waves; tw0[nk] = waves used)
+        uint32_t wq[NW], tw0[NW + 1], tkw[NW];
         uint16_t cnt[NW][NBIN];   // per-wave digit counts, then the digits' first slots
         U wmin[NW], wmax[NW];
         U tmin[NW];
@@ -855,25 +870,23 @@ struct LevelK {
         return x;
     }
 
-    // One workgroup step of a radix class: nk tasks of kw waves each; wave w
-    // holds items i at task position (w % kw)*WS + i*WAVE + lane, read from
-    // global memory straight into registers.  The step's LDS arrays are in
-    // slot order (task t at slots [t*kw*WS, (t+1)*kw*WS)); on return they
-    // hold every task's keys sorted at the start of its slots.
-    // floor(x / kw) = (x * (2^32/kw + 1)) >> 32 in 64-bit arithmetic, exact
-    // for x < 2^15 and kw <= 8 (all slots and scan entries are below 2^13).
-    static __device__ void radix_step(const Params &p, U *sk, uint32_t *si, const TaskRec *rec, uint32_t kw,
-                                      uint32_t nk, uint32_t tid, RadixShared &rs)
+    // One workgroup step of the radix classes: nk tasks, task t holding the
+    // rs.tkw[t] consecutive waves from rs.tw0[t] (one class, or the mixed
+    // 3+3+2 steps); wave w holds items i at task position
+    // (w - tw0)*WS + i*WAVE + lane, read from global memory straight into
+    // registers.  The step's LDS arrays are in slot order (task t at slots
+    // [tw0[t]*WS, tw0[t+1]*WS)); on return they hold every task's keys
+    // sorted at the start of its slots.
+    static __device__ void radix_step(const Params &p, U *sk, uint32_t *si, const TaskRec *rec, uint32_t nk,
+                                      uint32_t tid, RadixShared &rs)
     {
         const U flip = (U)p.flip;
-        // floor(w / kw) = (w * M16[kw]) >> 16, M16[kw] = 2^16/kw + 1: exact
-        // for w < 8 and kw <= 8 (a table instead of a 64-bit division)
-        constexpr uint32_t M16[9] = {0u, 65537u, 32769u, 21846u, 16385u, 13108u, 10923u, 9363u, 8193u};
         static_assert(NW <= 8, "one workgroup step holds at most 8 waves");
         const uint32_t w = __builtin_amdgcn_readfirstlane(tid / WAVE), l = tid % WAVE;
-        const uint32_t q = (w * M16[kw < 9u ? kw : 8u]) >> 16;   // task of this wave
+        const uint32_t q = __builtin_amdgcn_readfirstlane(rs.wq[w]);   // task of this wave
         const bool act = q < nk;                          // waves past the step's tasks idle
-        const uint32_t j0 = (w - q * kw) * WS + l;        // task position of item i: j0 + i*WAVE
+        const uint32_t wb = act ? __builtin_amdgcn_readfirstlane(rs.tw0[q]) : 0u;   // task's first wave
+        const uint32_t j0 = (w - wb) * WS + l;            // task position of item i: j0 + i*WAVE
         const TaskRec tr = act ? rec[q] : TaskRec{0, 0, 0};
         U *const rk = sk + w * WS + l;                    // slot of item i: rk[i*WAVE]
         uint32_t *const ri = si + w * WS + l;
@@ -919,7 +932,7 @@ struct LevelK {
         block_sync();
         if (tid < nk) {
             U a = key_max<U>(), b = 0;
-            for (uint32_t x = tid * kw; x < (tid + 1) * kw; x++) {
+            for (uint32_t x = rs.tw0[tid]; x < rs.tw0[tid + 1]; x++) {
                 a = rs.wmin[x] < a ? rs.wmin[x] : a;
                 b = rs.wmax[x] > b ? rs.wmax[x] : b;
             }
@@ -931,18 +944,24 @@ struct LevelK {
         uint32_t npass = (rs.maxbits + RW - 1) / RW;
         npass = npass < MAXPASS ? npass : MAXPASS;
         const U tmin = act ? rs.tmin[q] : (U)0;
-        // scan entries per thread and their (pair, wave) nibbles, the same
-        // for every thread: entry k2 is pair k2 / kw, wave k2 % kw
-        static_assert(NBIN == (uint32_t)NT && EPT <= 8 && NW <= 15, "scan ownership");
-        const uint32_t nent = nk * kw;
+        // scan ownership: the step's entries in (task, digit, wave of the
+        // task) order are NBIN per wave in use, so thread tid owns the EPT
+        // consecutive entries from tid*EPT, all inside its own wave's task:
+        // entry e1 + k2 of the task (e1 = (w - wb)*NBIN + l*EPT) is digit
+        // (e1 + k2) / kq, wave wb + (e1 + k2) % kq; kept as nibbles from
+        // (d0, w0).  Threads of idle waves own none.
+        static_assert(NBIN == (uint32_t)NT && EPT <= 8 && NW <= 8 && NBIN % EPT == 0, "scan ownership");
+        const uint32_t kq = act ? __builtin_amdgcn_readfirstlane(rs.tkw[q]) : 1u;
+        const uint32_t e1 = (w - wb) * NBIN + l * EPT;
+        const uint32_t d0 = e1 / kq, w0 = e1 - d0 * kq;
         uint32_t ej = 0, ew = 0;
         {
-            uint32_t jj = 0, ww = 0;
+            uint32_t jj = 0, ww = w0;
 #pragma unroll
             for (int k2 = 0; k2 < (int)EPT; k2++) {
                 ej |= jj << (4 * k2);
                 ew |= ww << (4 * k2);
-                if (++ww == kw) {
+                if (++ww == kq) {
                     ww = 0;
                     jj++;
                 }
@@ -975,32 +994,24 @@ struct LevelK {
             }
             block_sync();
             // exclusive scan over (task, digit, wave of the task): the first
-            // slot of every (wave, digit) run of the step.  Thread tid owns
-            // the nk (task, digit) pairs tid*nk .. tid*nk+nk-1 (NBIN == NT),
-            // i.e. nk*kw <= EPT entries; entry k2 is pair k2/kw, wave k2%kw
-            // of the task (from the nibbles ej, ew)
+            // slot of every (wave, digit) run of the step
             {
                 uint32_t cv[EPT], loc = 0;
 #pragma unroll
                 for (int k2 = 0; k2 < (int)EPT; k2++) {
                     cv[k2] = 0u;
-                    if ((uint32_t)k2 < nent) {   // uniform
-                        const uint32_t td = tid * nk + ((ej >> (4 * k2)) & 15u);   // task*NBIN + digit
-                        const uint32_t row = (td >> RW) * kw + ((ew >> (4 * k2)) & 15u);
-                        cv[k2] = rs.cnt[row][td & (NBIN - 1)];
-                    }
+                    if (act)   // wave-uniform
+                        cv[k2] = rs.cnt[wb + ((ew >> (4 * k2)) & 15u)][d0 + ((ej >> (4 * k2)) & 15u)];
                     loc += cv[k2];
                 }
                 uint32_t tot;
                 uint32_t run = block_excl_scan<NT>(loc, rs.red, &tot);
+                if (act) {
 #pragma unroll
-                for (int k2 = 0; k2 < (int)EPT; k2++) {
-                    if ((uint32_t)k2 < nent) {
-                        const uint32_t td = tid * nk + ((ej >> (4 * k2)) & 15u);
-                        const uint32_t row = (td >> RW) * kw + ((ew >> (4 * k2)) & 15u);
-                        rs.cnt[row][td & (NBIN - 1)] = (uint16_t)run;
+                    for (int k2 = 0; k2 < (int)EPT; k2++) {
+                        rs.cnt[wb + ((ew >> (4 * k2)) & 15u)][d0 + ((ej >> (4 * k2)) & 15u)] = (uint16_t)run;
+                        run += cv[k2];
                     }
-                    run += cv[k2];
                 }
             }
             block_sync();
@@ -1038,7 +1049,7 @@ struct LevelK {
         const U flip = (U)p.flip;
         U *dst = (U *)p.kout;
         const Ctrl *cc = p.ctrl;
-        if (threadIdx.x == 0) lds_plan(cc, CPLAN, NW, CAP, rs.coff, rs.cwg);
+        if (threadIdx.x == 0) lds_plan(cc, CPLAN, NW, CAP, rs.coff, rs.cwg, &rs.mix);
         block_sync();
         // merge classes [CMIN, CPLAN), radix classes [CRAD, CLAST] (none when
         // MERGE_ONLY: the merge classes then reach CMAX)
@@ -1057,21 +1068,31 @@ struct LevelK {
             uint32_t c = c0;
             while (c < clast && v >= rs.cwg[c + 1]) c++;
             const uint32_t kw = RAD ? c - CRAD + 1 : 0u;
-            const uint32_t region = RAD ? kw * WS : 1u << c;     // slots per task
+            const uint32_t region = RAD ? kw * WS : 1u << c;     // slots per task of the class
             const uint32_t P = RAD ? (uint32_t)NW / (kw ? kw : 1u) : CAP >> c;
-            const uint32_t first = (v - rs.cwg[c]) * P;
+            const uint32_t sv = v - rs.cwg[c];                   // step inside the class
+            // the first rs.mix tasks of the 2-wave class ride in the mixed
+            // steps (the first rs.mix steps of the 3-wave class), one each
+            const bool mixed = RAD && kw == 3 && sv < rs.mix;
+            const uint32_t first = sv * P + (RAD && kw == 2 ? rs.mix : 0u);
             const uint32_t cnt = cc->cls_cnt[c];
-            const uint32_t nk = first < cnt ? ((cnt - first) < P ? (cnt - first) : P) : 0u;
-            for (uint32_t q = threadIdx.x; q < P; q += NT) {
+            const uint32_t nkc = first < cnt ? ((cnt - first) < P ? (cnt - first) : P) : 0u;
+            const uint32_t nk = nkc + (mixed ? 1u : 0u);
+            // every record the step reads is set: the merge path reads all P
+            // (tasks past nk have length 0), the radix path the nk tasks
+            const uint32_t nrec = RAD ? nk : P;
+            for (uint32_t q = threadIdx.x; q < nrec; q += NT) {
                 TaskRec r{0, 0, 0};
                 if (q < nk) {
-                    const uint32_t slot = rs.coff[c] + first + q;
+                    const bool part = q == nkc && mixed;             // the mixed step's 2-wave task
+                    const uint32_t slot = part ? rs.coff[c - 1] + sv : rs.coff[c] + first + q;
+                    const uint32_t reg = part ? 2 * WS : region;
                     const uint32_t k = slot < p.ldscap ? p.ldsq[slot] : ~0u;
                     if (k < p.ldscap) {
                         const LdsTask tk = p.ldst[k];
                         r = TaskRec{tk.begin, tk.len, tk.depth};
                     }
-                    if (!span_ok(p, r.begin, r.len) || r.len > region) {
+                    if (!span_ok(p, r.begin, r.len) || r.len > reg) {
                         range_fail(p, 1);
                         r.len = 0;
                     }
@@ -1080,21 +1101,30 @@ struct LevelK {
                 }
                 rec[q] = r;
             }
+            if (RAD && threadIdx.x == 0) {
+                // waves of the step's tasks, in task order
+                uint32_t W = 0;
+                for (uint32_t t = 0; t < nk; t++) {
+                    const uint32_t kt = t < nkc ? kw : 2u;
+                    rs.tkw[t] = kt;
+                    rs.tw0[t] = W;
+                    for (uint32_t x = 0; x < kt && W < (uint32_t)NW; x++) rs.wq[W++] = t;
+                }
+                rs.tw0[nk] = W;
+                for (uint32_t x = W; x < (uint32_t)NW; x++) rs.wq[x] = NW;   // idle
+            }
             block_sync();
             if constexpr (RAD) {
-                radix_step(p, sk, si, rec, kw, nk, tid, rs);
-                // write-out: task t's sorted keys are at slots [t*region, t*region + len)
+                radix_step(p, sk, si, rec, nk, tid, rs);
+                // write-out: task t's sorted keys are at slots [tw0[t]*WS, tw0[t]*WS + len)
                 uint32_t tid2 = threadIdx.x;
                 asm volatile("" : "+v"(tid2));
-                // floor(x / kw) = (x * M16[kw]) >> 16 for x < 8 (slot wave x)
-                constexpr uint32_t M16[9] = {0u, 65537u, 32769u, 21846u, 16385u, 13108u, 10923u, 9363u, 8193u};
-                const uint32_t m16 = M16[kw < 9u ? kw : 8u];
 #pragma unroll
                 for (int r = 0; r < SI; r++) {
                     const uint32_t s = tid2 + (uint32_t)r * NT;
-                    const uint32_t t = ((s / WS) * m16) >> 16;
-                    const uint32_t j = s - t * region;
+                    const uint32_t t = rs.wq[s / WS];
                     if (t < nk) {
+                        const uint32_t j = s - rs.tw0[t] * WS;
                         const TaskRec tr = rec[t];
                         if (j < tr.len) {
                             dst[tr.begin + j] = from_ord<U, FL>(sk[s], flip);
@@ -1402,6 +1432,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k
 {
     typedef LevelK<KT, IDX, S> K;
     __shared__ __align__(16) unsigned char arena[Cfg<KT, IDX, S>::SORT_BYTES];
+    static_assert((Cfg<KT, IDX, S>::CAP >> Cfg<KT, IDX, S>::CMIN) > Cfg<KT, IDX, S>::NW, "task records of a step");
     __shared__ typename K::TaskRec rec[Cfg<KT, IDX, S>::CAP >> Cfg<KT, IDX, S>::CMIN];
     __shared__ typename K::RadixShared rs;
     __shared__ unsigned long long acc[2];
