@@ -155,3 +155,33 @@ def test_segmented_level_batches(monkeypatch, orc, torch_dev):
     assert st["error"] == 0 and st["max_depth"] >= 3
     check_segments(orc, keys, off, d2h(torch, kout), d2h(torch, iout))
     sorter.close()
+
+
+@pytest.mark.parametrize("n2,n3", [(0, 5), (1, 5), (3, 3), (2, 4), (7, 1), (5, 0), (4, 9)])
+def test_radix_mixed_steps(orc, torch_dev, n2, n3):
+    """Radix-class tasks of 2 waves (1025..2048 keys) and 3 waves
+    (2049..3072 keys) in one batch: the first min(ceil(n3/2), n2) steps
+    of the 3-wave class also sort one 2-wave task each (an odd last step
+    holds one 3-wave task and its 2-wave partner); the rest run in steps
+    of their own class.  Plus a few 1-wave and 4-wave tasks, shuffled."""
+    torch = torch_dev
+    rng = np.random.default_rng(1000 + 10 * n2 + n3)
+    lens = np.concatenate([rng.integers(1025, 2049, n2), rng.integers(2049, 3073, n3),
+                           rng.integers(513, 1025, 3), rng.integers(3073, 4097, 2)]).astype(np.int64)
+    rng.shuffle(lens)
+    off = wl.segment_offsets(lens)
+    n = int(off[-1])
+    sorter = TurtleSort(np.int32, n)
+    sorter.set_tuning(lds_cutoff=8192)
+    for shift in (0, 40):
+        print(f"case n2={n2} n3={n3} shift={shift}", flush=True)
+        keys = wl.gen_keys(np.int32, n, 7 * n2 + n3 + shift, shift)
+        kin = h2d(torch, keys)
+        iin = torch.arange(n, dtype=torch.int32, device="cuda")
+        kout = torch.empty_like(kin)
+        iout = torch.empty_like(iin)
+        sorter.segmented_sort_device(kin, kout, off, iin, iout)
+        st = sorter.synchronize()
+        assert st["error"] == 0 and st["segments"] == 0
+        check_segments(orc, keys, off, d2h(torch, kout), d2h(torch, iout))
+    sorter.close()
