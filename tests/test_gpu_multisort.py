@@ -14,8 +14,8 @@ pytestmark = pytest.mark.gpu
 @pytest.mark.parametrize("dtype", [np.int8, np.uint8, np.int16, np.uint16, np.int32, np.uint32, np.int64, np.uint64])
 def test_multisort_host_api(orc, torch_dev, dtype):
     sorter = TurtleSort(dtype, 1 << 20)
-    for arr, num in [(1, 7), (2, 33), (10, 640), (32, 100), (1000, 37), (1024, 256), (2038, 50), (4096, 9),
-                     (8192, 3), (8193, 2), (20000, 3)]:
+    for arr, num in [(1, 7), (2, 33), (10, 640), (32, 100), (300, 21), (512, 40), (1000, 37), (1024, 256),
+                     (1500, 11), (2038, 50), (2048, 13), (4096, 9), (8192, 3), (8193, 2), (20000, 3)]:
         print(f"case dtype={np.dtype(dtype)} arr={arr} num={num}", flush=True)
         d = wl.gen_keys(dtype, arr * num, arr * 7 + num, 0)
         expect = orc.multisort(d, arr)

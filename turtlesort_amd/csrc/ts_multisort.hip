@@ -74,12 +74,22 @@ static hipError_t ms_launch(uint64_t flip, uint32_t arr_size, const void *in, vo
         U *oo = o + e0;
         const uint32_t *xi = iin ? iin + e0 : nullptr;
         uint32_t *xo = IDX ? iout + e0 : nullptr;
+        // one workgroup per array, sized so that every thread holds ITEMS
+        // keys of it (one wave up to 1024 keys: no cross-wave barriers, and
+        // 16 keys per thread take one merge round fewer than 8; 65,536 x
+        // 1024 int32 keys: 0.34 -> 0.29 ms)
         if (ne <= 256)
             hipLaunchKernelGGL((ms_kernel<U, FL, IDX, 64, 4>), dim3((uint32_t)na), dim3(64), 0, st, ii, oo, xi, xo,
                                arr_size, f);
-        else if (ne <= 1024)
-            hipLaunchKernelGGL((ms_kernel<U, FL, IDX, 128, 8>), dim3((uint32_t)na), dim3(128), 0, st, ii, oo, xi, xo,
+        else if (ne <= 512)
+            hipLaunchKernelGGL((ms_kernel<U, FL, IDX, 64, 8>), dim3((uint32_t)na), dim3(64), 0, st, ii, oo, xi, xo,
                                arr_size, f);
+        else if (ne <= 1024)
+            hipLaunchKernelGGL((ms_kernel<U, FL, IDX, 64, 16>), dim3((uint32_t)na), dim3(64), 0, st, ii, oo, xi, xo,
+                               arr_size, f);
+        else if (ne <= 2048)
+            hipLaunchKernelGGL((ms_kernel<U, FL, IDX, 128, 16>), dim3((uint32_t)na), dim3(128), 0, st, ii, oo, xi,
+                               xo, arr_size, f);
         else if (ne <= 4096)
             hipLaunchKernelGGL((ms_kernel<U, FL, IDX, 256, 16>), dim3((uint32_t)na), dim3(256), 0, st, ii, oo, xi,
                                xo, arr_size, f);
