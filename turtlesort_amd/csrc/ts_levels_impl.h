@@ -840,6 +840,8 @@ struct LevelK {
     struct PivotShared {
         uint16_t cnt[NW][NBIN];       // per-wave digit counts, then the digits' first positions
         U buf[NW][C::SAMPLE];         // per-wave sample
+        uint32_t ntl[NW];             // tiles of each wave's segment in this step
+        uint32_t tbase, done;         // the step's first tile; segments done
     };
 
     template <typename T>
@@ -1229,134 +1231,156 @@ struct LevelK {
         U *const b = ps.buf[w];
         uint16_t *const cn = ps.cnt[w];
         uint32_t done = 0;
+        if (threadIdx.x == 0) ps.done = 0;
+        block_sync();
         // segment sb + w*grid + block (+ k*grid*NW): a level with fewer
         // segments than waves in the grid spreads over every workgroup, one
-        // wave per SIMD, instead of filling the first workgroups' 8 waves
-        for (uint32_t sid = lv.sb + w * gridDim.x + blockIdx.x; sid < lv.se; sid += gridDim.x * NW) {   // wave-uniform
-            done++;
-            Seg *sg = &p.segs[sid];
-            const uint32_t begin = sg->begin, len = sg->len, depth = sg->depth;
-            if (!span_ok(p, begin, len)) {
-                if (l == 0) {
-                    range_fail(p, 3);
-                    sg->ntiles = 0;
-                    sg->tile_base = 0;
-                }
-                continue;
-            }
-            const uint32_t m = len < SAMPLE ? len : SAMPLE;
-            const U *src = src_keys<U>(p, depth) + begin;
-            U k[SPL];
-            U mn = key_max<U>(), mx = 0;
-            // len >= SAMPLE: SAMPLE/RUN runs of RUN consecutive keys, run r
-            // starting at r*st + (st - RUN)/2 with st = len / (SAMPLE/RUN), so
-            // that a wave touches one line per run rather than one per key;
-            // sample j = i*WAVE + l is key j % RUN of run j / RUN.  The last
-            // run ends before len: st >= RUN and (SAMPLE/RUN)*st <= len.
-            // Else every key.
-            constexpr uint32_t RUN = 8, NRUN = SAMPLE / RUN;
-            static_assert(SAMPLE >= 8 * RUN && (WAVE % RUN) == 0, "sample runs");
-            const uint32_t st = len / NRUN;
-            const uint32_t base = m == SAMPLE ? (l / RUN) * st + (st - RUN) / 2 + l % RUN : l;
-            const uint32_t step = m == SAMPLE ? (WAVE / RUN) * st : WAVE;
-            // all loads first (positions past the sample re-read key 0),
-            // then the predicates, so that the loads' latencies overlap
-#pragma unroll
-            for (int i = 0; i < (int)SPL; i++) {
-                const uint32_t j = (uint32_t)i * WAVE + l;
-                k[i] = src[j < m ? base + (uint32_t)i * step : 0u];
-            }
-#pragma unroll
-            for (int i = 0; i < (int)SPL; i++) {
-                const bool real = (uint32_t)i * WAVE + l < m;
-                k[i] = real ? to_ord<U, FL>(k[i], flip) : key_max<U>();
-                mn = real && k[i] < mn ? k[i] : mn;
-                mx = real && k[i] > mx ? k[i] : mx;
-            }
-            mn = wave_min(mn);
-            mx = wave_max(mx);
-            // The sample is sorted on the top 2*RW bits of its span only (at
-            // most two passes): the keys at the quantile ranks are then within
-            // 2^-18 of the span of the exact quantiles, and the picked
-            // splitters are put in exact order below.
-            const uint64_t span = mx > mn ? (uint64_t)(U)(mx - mn) : 0ull;
-            const uint32_t bits = span ? (uint32_t)(64 - __clzll(span)) : 0u;
-            const uint32_t low = bits > 2u * RW ? bits - 2u * RW : 0u;   // bits left unsorted
-            const uint32_t npass = (bits - low + RW - 1) / RW;
-            for (uint32_t pass = 0; pass < npass; pass++) {
-                const uint32_t sh = low + pass * RW;
-                for (uint32_t d = l; d < NBIN; d += WAVE) cn[d] = 0;
-                wave_sync();
-                uint32_t dr[SPL];
-#pragma unroll
-                for (int i = 0; i < (int)SPL; i++) {
-                    // after a pass the sample's keys are at its first m positions
-                    const bool real = (uint32_t)i * WAVE + l < m;
-                    const uint32_t d = real ? (uint32_t)((U)(k[i] - mn) >> sh) & (NBIN - 1) : NBIN - 1;
-                    const uint64_t peers = match_peers<RW>(d, true);
-                    const uint32_t r = mask_rank(peers);
-                    const uint32_t old = cn[d];
-                    if (r == 0) cn[d] = (uint16_t)(old + (uint32_t)__popcll(peers));
-                    dr[i] = d | ((old + r) << 16);
-                }
-                wave_sync();
-                // exclusive scan of the NBIN digit counts: lane l owns NBIN/WAVE digits
-                {
-                    constexpr uint32_t DPL = NBIN / WAVE;
-                    uint32_t c8[DPL], loc = 0;
-#pragma unroll
-                    for (uint32_t q = 0; q < DPL; q++) {
-                        c8[q] = cn[l * DPL + q];
-                        loc += c8[q];
+        // wave per SIMD, instead of filling the first workgroups' 8 waves.
+        // The steps are block-uniform: the workgroup reserves the tiles of
+        // its step's segments with one atomic (a level of 1024 segments
+        // otherwise queues 1024 atomics on one word).
+        for (uint32_t s0 = lv.sb + blockIdx.x; s0 < lv.se; s0 += gridDim.x * NW) {   // block-uniform
+            const uint32_t sid = s0 + w * gridDim.x;
+            Seg *sg = &p.segs[sid < lv.se ? sid : s0];
+            uint32_t ntiles = 0;
+            if (sid < lv.se) {   // wave-uniform
+                done++;
+                const uint32_t begin = sg->begin, len = sg->len, depth = sg->depth;
+                if (!span_ok(p, begin, len)) {
+                    if (l == 0) {
+                        range_fail(p, 3);
+                        sg->ntiles = 0;
+                        sg->tile_base = 0;
                     }
-                    uint32_t run = wave_incl_scan(loc) - loc;
-#pragma unroll
-                    for (uint32_t q = 0; q < DPL; q++) {
-                        cn[l * DPL + q] = (uint16_t)run;
-                        run += c8[q];
+                } else {
+                    const uint32_t m = len < SAMPLE ? len : SAMPLE;
+                    const U *src = src_keys<U>(p, depth) + begin;
+                    U k[SPL];
+                    U mn = key_max<U>(), mx = 0;
+                    // len >= SAMPLE: SAMPLE/RUN runs of RUN consecutive keys, run r
+                    // starting at r*st + (st - RUN)/2 with st = len / (SAMPLE/RUN), so
+                    // that a wave touches one line per run rather than one per key;
+                    // sample j = i*WAVE + l is key j % RUN of run j / RUN.  The last
+                    // run ends before len: st >= RUN and (SAMPLE/RUN)*st <= len.
+                    // Else every key.
+                    constexpr uint32_t RUN = 8, NRUN = SAMPLE / RUN;
+                    static_assert(SAMPLE >= 8 * RUN && (WAVE % RUN) == 0, "sample runs");
+                    const uint32_t st = len / NRUN;
+                    const uint32_t base = m == SAMPLE ? (l / RUN) * st + (st - RUN) / 2 + l % RUN : l;
+                    const uint32_t step = m == SAMPLE ? (WAVE / RUN) * st : WAVE;
+                    // all loads first (positions past the sample re-read key 0),
+                    // then the predicates, so that the loads' latencies overlap
+        #pragma unroll
+                    for (int i = 0; i < (int)SPL; i++) {
+                        const uint32_t j = (uint32_t)i * WAVE + l;
+                        k[i] = src[j < m ? base + (uint32_t)i * step : 0u];
                     }
+        #pragma unroll
+                    for (int i = 0; i < (int)SPL; i++) {
+                        const bool real = (uint32_t)i * WAVE + l < m;
+                        k[i] = real ? to_ord<U, FL>(k[i], flip) : key_max<U>();
+                        mn = real && k[i] < mn ? k[i] : mn;
+                        mx = real && k[i] > mx ? k[i] : mx;
+                    }
+                    mn = wave_min(mn);
+                    mx = wave_max(mx);
+                    // The sample is sorted on the top 2*RW bits of its span only (at
+                    // most two passes): the keys at the quantile ranks are then within
+                    // 2^-18 of the span of the exact quantiles, and the picked
+                    // splitters are put in exact order below.
+                    const uint64_t span = mx > mn ? (uint64_t)(U)(mx - mn) : 0ull;
+                    const uint32_t bits = span ? (uint32_t)(64 - __clzll(span)) : 0u;
+                    const uint32_t low = bits > 2u * RW ? bits - 2u * RW : 0u;   // bits left unsorted
+                    const uint32_t npass = (bits - low + RW - 1) / RW;
+                    for (uint32_t pass = 0; pass < npass; pass++) {
+                        const uint32_t sh = low + pass * RW;
+                        for (uint32_t d = l; d < NBIN; d += WAVE) cn[d] = 0;
+                        wave_sync();
+                        uint32_t dr[SPL];
+        #pragma unroll
+                        for (int i = 0; i < (int)SPL; i++) {
+                            // after a pass the sample's keys are at its first m positions
+                            const bool real = (uint32_t)i * WAVE + l < m;
+                            const uint32_t d = real ? (uint32_t)((U)(k[i] - mn) >> sh) & (NBIN - 1) : NBIN - 1;
+                            const uint64_t peers = match_peers<RW>(d, true);
+                            const uint32_t r = mask_rank(peers);
+                            const uint32_t old = cn[d];
+                            if (r == 0) cn[d] = (uint16_t)(old + (uint32_t)__popcll(peers));
+                            dr[i] = d | ((old + r) << 16);
+                        }
+                        wave_sync();
+                        // exclusive scan of the NBIN digit counts: lane l owns NBIN/WAVE digits
+                        {
+                            constexpr uint32_t DPL = NBIN / WAVE;
+                            uint32_t c8[DPL], loc = 0;
+        #pragma unroll
+                            for (uint32_t q = 0; q < DPL; q++) {
+                                c8[q] = cn[l * DPL + q];
+                                loc += c8[q];
+                            }
+                            uint32_t run = wave_incl_scan(loc) - loc;
+        #pragma unroll
+                            for (uint32_t q = 0; q < DPL; q++) {
+                                cn[l * DPL + q] = (uint16_t)run;
+                                run += c8[q];
+                            }
+                        }
+                        wave_sync();
+        #pragma unroll
+                        for (int i = 0; i < (int)SPL; i++) b[(cn[dr[i] & 0xffffu] + (dr[i] >> 16)) & (SAMPLE - 1)] = k[i];
+                        wave_sync();
+        #pragma unroll
+                        for (int i = 0; i < (int)SPL; i++) k[i] = b[(uint32_t)i * WAVE + l];
+                        wave_sync();
+                    }
+                    if (npass == 0) {
+        #pragma unroll
+                        for (int i = 0; i < (int)SPL; i++) b[(uint32_t)i * WAVE + l] = k[i];
+                        wave_sync();
+                    }
+                    // splitter l = the key at quantile rank (l+1) m / (S+1), then each
+                    // lane's rank among the S picked keys (ties by lane) orders them
+                    // exactly (they are already ordered when the sort was exact)
+                    const U spv = l < (uint32_t)S ? b[((l + 1) * m) / (S + 1)] : key_max<U>();
+                    uint32_t srank = 0;
+        #pragma unroll
+                    for (int t = 0; t < S; t++) {
+                        const U o = shfl_u<U>(spv, t);
+                        srank += (o < spv || (o == spv && (uint32_t)t < l)) ? 1u : 0u;
+                    }
+                    if (l < (uint32_t)S) sg->split[srank] = (uint64_t)spv;
+                    ntiles = (len + TILE - 1) / TILE;
                 }
-                wave_sync();
-#pragma unroll
-                for (int i = 0; i < (int)SPL; i++) b[(cn[dr[i] & 0xffffu] + (dr[i] >> 16)) & (SAMPLE - 1)] = k[i];
-                wave_sync();
-#pragma unroll
-                for (int i = 0; i < (int)SPL; i++) k[i] = b[(uint32_t)i * WAVE + l];
-                wave_sync();
             }
-            if (npass == 0) {
-#pragma unroll
-                for (int i = 0; i < (int)SPL; i++) b[(uint32_t)i * WAVE + l] = k[i];
-                wave_sync();
-            }
-            // splitter l = the key at quantile rank (l+1) m / (S+1), then each
-            // lane's rank among the S picked keys (ties by lane) orders them
-            // exactly (they are already ordered when the sort was exact)
-            const U spv = l < (uint32_t)S ? b[((l + 1) * m) / (S + 1)] : key_max<U>();
-            uint32_t srank = 0;
-#pragma unroll
-            for (int t = 0; t < S; t++) {
-                const U o = shfl_u<U>(spv, t);
-                srank += (o < spv || (o == spv && (uint32_t)t < l)) ? 1u : 0u;
-            }
-            if (l < (uint32_t)S) sg->split[srank] = (uint64_t)spv;
-            const uint32_t ntiles = (len + TILE - 1) / TILE;
-            uint32_t tb = 0;
-            if (l == 0) {
-                tb = atomic_add(&p.ctrl->tile_alloc, ntiles);
-                if (tb > p.tilecap || ntiles > p.tilecap - tb) {
+            if (l == 0) ps.ntl[w] = ntiles;
+            block_sync();
+            if (threadIdx.x == 0) {
+                uint32_t tot = 0;
+                for (uint32_t x = 0; x < (uint32_t)NW; x++) tot += ps.ntl[x];
+                uint32_t tb = tot ? atomic_add(&p.ctrl->tile_alloc, tot) : 0u;
+                if (tot && (tb > p.tilecap || tot > p.tilecap - tb)) {
                     atomic_or(&p.ctrl->error, ERR_TILES);
                     tb = ~0u;
                 }
-                sg->ntiles = tb == ~0u ? 0u : ntiles;
-                sg->tile_base = tb == ~0u ? 0u : tb;
+                ps.tbase = tb;
             }
-            tb = __shfl(tb, 0, WAVE);
-            if (tb != ~0u)
-                for (uint32_t t = l; t < ntiles; t += WAVE) p.tile_seg[tb + t] = sid;
-            wave_sync();
+            block_sync();
+            if (ntiles) {   // wave-uniform
+                uint32_t tb = ps.tbase;
+                if (tb != ~0u)
+                    for (uint32_t x = 0; x < w; x++) tb += ps.ntl[x];
+                if (l == 0) {
+                    sg->ntiles = tb == ~0u ? 0u : ntiles;
+                    sg->tile_base = tb == ~0u ? 0u : tb;
+                }
+                if (tb != ~0u)
+                    for (uint32_t t = l; t < ntiles; t += WAVE) p.tile_seg[tb + t] = sid;
+            }
+            block_sync();   // the next step rewrites ps.ntl
         }
-        if (l == 0 && done) atomic_add64((unsigned long long *)&p.ctrl->tasks[T_PIVOT], (unsigned long long)done);
+        if (l == 0 && done) atomicAdd(&ps.done, done);
+        block_sync();
+        if (threadIdx.x == 0 && ps.done) atomic_add64((unsigned long long *)&p.ctrl->tasks[T_PIVOT], (unsigned long long)ps.done);
     }
 };
 
