@@ -758,32 +758,30 @@ struct LevelK {
     }
 
     // task ids ordered by class (order inside a class is arbitrary; every
-    // task is sorted on its own, so the output does not depend on it)
+    // task is sorted on its own, so the output does not depend on it).  Each
+    // workgroup takes a contiguous chunk of the tasks: it counts the chunk's
+    // classes, reserves their slots with one atomic per class, then places
+    // the chunk's tasks (reserving per class and 256 tasks made ~84K
+    // atomics on 20 words for 1.66M tasks).
     static __device__ void ldsbucket(const Params &p, uint32_t *hist, uint32_t *wbase, uint32_t *coff, uint32_t *cwg)
     {
         const uint32_t ntask = p.ctrl->lds_alloc < p.ldscap ? p.ctrl->lds_alloc : p.ldscap;
-        const uint32_t stride = gridDim.x * blockDim.x;
+        const uint32_t per = (ntask + gridDim.x - 1) / gridDim.x;
+        const uint32_t a = blockIdx.x * per < ntask ? blockIdx.x * per : ntask;
+        const uint32_t b = ntask - a < per ? ntask : a + per;
         if (threadIdx.x == 0) lds_plan(p.ctrl, CPLAN, NW, CAP, coff, cwg, nullptr);
+        if (threadIdx.x < (uint32_t)LDS_CLASSES) hist[threadIdx.x] = 0;
         block_sync();
-        for (uint32_t k0 = blockIdx.x * blockDim.x; k0 < ntask; k0 += stride) {
-            if (threadIdx.x < (uint32_t)LDS_CLASSES) hist[threadIdx.x] = 0;
-            block_sync();
-            const uint32_t k = k0 + threadIdx.x;
-            uint32_t c = 0, lr = 0;
-            if (k < ntask) {
-                c = lds_class(p.ldst[k].len);
-                lr = atomicAdd(&hist[c], 1u);
-            }
-            block_sync();
-            if (threadIdx.x < (uint32_t)LDS_CLASSES && hist[threadIdx.x])
-                wbase[threadIdx.x] = atomic_add(&p.ctrl->cls_cur[threadIdx.x], hist[threadIdx.x]);
-            block_sync();
-            if (k < ntask) {
-                const uint32_t slot = coff[c] + wbase[c] + lr;
-                if (slot < p.ldscap) p.ldsq[slot] = k;
-                else range_fail(p, 11);
-            }
-            block_sync();
+        for (uint32_t k = a + threadIdx.x; k < b; k += blockDim.x) atomicAdd(&hist[lds_class(p.ldst[k].len)], 1u);
+        block_sync();
+        if (threadIdx.x < (uint32_t)LDS_CLASSES)
+            wbase[threadIdx.x] = hist[threadIdx.x] ? atomic_add(&p.ctrl->cls_cur[threadIdx.x], hist[threadIdx.x]) : 0u;
+        block_sync();
+        for (uint32_t k = a + threadIdx.x; k < b; k += blockDim.x) {
+            const uint32_t c = lds_class(p.ldst[k].len);
+            const uint32_t slot = coff[c] + atomicAdd(&wbase[c], 1u);   // LDS: a distinct slot per task
+            if (slot < p.ldscap) p.ldsq[slot] = k;
+            else range_fail(p, 11);
         }
     }
 
