@@ -87,7 +87,8 @@ typedef struct turtle_tuning {
 int turtle_create(int dtype, int64_t max_n, int track_index, int compress, turtle_handle *out);
 
 /* Replaces ~TurtleSort (turtle-sort.cuh:278-282) and the buffer frees of
- * TurtleBuffer::~Buffer (turtle-buffer.cuh:82-99). */
+ * TurtleBuffer::~Buffer (turtle-buffer.cuh:82-99).  TURTLE_ERR_STATE (and
+ * nothing freed) while graphs captured on the handle are alive. */
 int turtle_destroy(turtle_handle h);
 
 /* Replaces MemoryCompressionSuccessful (turtle-sort.cuh:115-118); always 0. */
@@ -95,9 +96,10 @@ int turtle_compression_enabled(turtle_handle h);
 
 /* Replaces StartSorting(std::vector<Type>*, std::vector<int>*)
  * (turtle-sort.cuh:124-184).  keys/idx are host arrays of n elements; idx
- * may be NULL.  Starts the host->device copy and the device sort on the
- * handle's stream and returns.  The caller keeps both arrays alive and
- * unchanged until turtle_sort_sync. */
+ * may be NULL.  Copies them to the device through pinned staging (the call
+ * takes about as long as that copy), queues the device sort on the handle's
+ * stream and returns without waiting for it.  The caller keeps both arrays
+ * alive and unchanged until turtle_sort_sync. */
 int turtle_sort_start(turtle_handle h, void *keys, int32_t *idx, int64_t n);
 
 /* Replaces Sync() (turtle-sort.cuh:188-258): waits, copies the sorted keys
@@ -139,9 +141,10 @@ int turtle_multisort_index(turtle_handle h, int dtype, int64_t arr_size, int64_t
  * idx_out when both are non-NULL; the handle must track indices).  keys_in
  * is not modified; keys_out may equal keys_in.  stream is a hipStream_t;
  * NULL is the null (default) stream, so the work is ordered after whatever
- * produced the inputs there.  Returns once the work is queued; the host
- * waits for a small read-back after the partition levels (the final LDS
- * sort may still be running when it returns). */
+ * produced the inputs there.  Asynchronous: returns once the work is queued
+ * (it first resolves the handle's previous run, see turtle_synchronize).
+ * The caller keeps keys_in/idx_in unchanged until turtle_synchronize, which
+ * makes the output final. */
 int turtle_sort_device(turtle_handle h, const void *keys_in, const int32_t *idx_in,
                        void *keys_out, int32_t *idx_out, int64_t n, void *stream);
 
@@ -172,9 +175,14 @@ int turtle_multisort_index_device(turtle_handle h, int dtype, int64_t arr_size, 
 /* hipGraph form of turtle_segmented_sort_device: the reset, root, level
  * and LDS-sort launches are captured once for a fixed number of levels
  * (offsets copied to a device buffer owned by the graph; no memset nodes)
- * and replayed by turtle_graph_launch on any stream (NULL = null stream).  The graph uses the handle's engine
- * tables, so at most one run per handle may be in flight.  keys_out must
- * differ from keys_in.  No reference counterpart. */
+ * and replayed by turtle_graph_launch on any stream (NULL = null stream).  A graph owns its engine
+ * tables (later calls on the handle never resize them) but shares the
+ * handle's key/index ping-pong buffers, so at most one run per handle may
+ * be in flight.  The graph captures the estimated levels plus two; if an
+ * input needs more, turtle_synchronize after the replay reports
+ * TURTLE_ERR_QUEUE_OVERFLOW (the output is then incomplete).  keys_out must
+ * differ from keys_in.  Destroy a handle's graphs before the handle.  No
+ * reference counterpart. */
 typedef struct turtle_graph *turtle_graph_handle;
 int turtle_segmented_graph_create(turtle_handle h, const void *keys_in, const int32_t *idx_in,
                                   void *keys_out, int32_t *idx_out, const int64_t *offsets,
@@ -182,8 +190,21 @@ int turtle_segmented_graph_create(turtle_handle h, const void *keys_in, const in
 int turtle_graph_launch(turtle_graph_handle g, void *stream);
 int turtle_graph_destroy(turtle_graph_handle g);
 
-/* Waits for the handle's last device work and fills *st (may be NULL). */
+/* Waits for the handle's last device work and fills *st (may be NULL).
+ * An engine run is queued with the partition levels its input size
+ * suggests; this call also finishes runs that need more (it queues further
+ * levels), and after a work-table overflow it grows the tables and sorts
+ * again from the input, which no level writes (an in-place call's input is
+ * kept in the handle's input buffer).  The next call on the handle does the
+ * same for the run before it.  The output is final once this returns OK. */
 int turtle_synchronize(turtle_handle h, turtle_stats *st);
+
+/* Non-blocking status of the handle's last device work, as hipStreamQuery
+ * (no reference counterpart: the reference's only probe is Sync): 1 while
+ * it still runs, 0 when it is done (call turtle_synchronize before reading
+ * the output, which also resolves its level check), or a negative status
+ * code on error. */
+int turtle_query(turtle_handle h);
 
 /* Kernels of one engine run (index of turtle_profile's arrays). */
 typedef enum turtle_kernel_kind {

@@ -121,6 +121,7 @@ SIGNATURES = {
     "turtle_multisort_index": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _i32, ctypes.POINTER(ctypes.c_double)]),
     "turtle_multisort_index_device": (_i32, [_vp, _i32, _i64, _vp, _vp, _vp, _vp, _i64, _vp]),
     "turtle_synchronize": (_i32, [_vp, ctypes.POINTER(Stats)]),
+    "turtle_query": (_i32, [_vp]),
     "turtle_segmented_graph_create": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, ctypes.POINTER(_vp)]),
     "turtle_graph_launch": (_i32, [_vp, _vp]),
     "turtle_graph_destroy": (_i32, [_vp]),

@@ -13,6 +13,7 @@ Errors raise TurtleError instead of exiting the process (the reference calls
 exit() from gpuErrchk, turtle-globals.cuh:18-30).
 """
 import ctypes
+import weakref
 
 import numpy as np
 
@@ -56,6 +57,70 @@ def _ptr(t):
     return None if t is None else ctypes.c_void_p(t.data_ptr())
 
 
+def _check_tensors(specs):
+    """Argument checks of the device forms, before any pointer reaches the C
+    ABI.  specs: (tensor, name, key type code or None, min_numel, is_index).
+    Every tensor's element type (int32 for an index; for keys the key
+    width of the handle's type, since keys may be passed as a same-width
+    bit view, e.g. uint32 keys in an int32 tensor), length and contiguity
+    are checked first, then that all are on the GPU.  Raises TypeError or
+    ValueError."""
+    import torch
+    for t, name, code, min_numel, index in specs:
+        if not isinstance(t, torch.Tensor):
+            raise TypeError(f"{name} must be a torch tensor")
+        if index:
+            if t.dtype != torch.int32:
+                raise TypeError(f"{name} must be int32, got {t.dtype}")
+        elif code is not None and t.element_size() != CODE_TO_NP[code].itemsize:
+            raise TypeError(f"{name} has dtype {t.dtype}, the sorter's key type is {CODE_TO_NP[code]}")
+        if t.numel() < min_numel:
+            raise ValueError(f"{name} holds {t.numel()} elements, needs at least {min_numel}")
+        if not t.is_contiguous():
+            raise ValueError(f"{name} must be contiguous")
+    for t, name, _, _, _ in specs:
+        if not t.is_cuda:
+            raise ValueError(f"{name} must be a GPU tensor")
+
+
+def validate_sort_args(key_code, track_index, n, keys_in, keys_out, idx_in=None, idx_out=None):
+    """checks of a device sort over the first n elements of keys_in"""
+    if (idx_in is None) != (idx_out is None):
+        raise ValueError("idx_in and idx_out must both be given or both be None")
+    if idx_in is not None and not track_index:
+        raise ValueError("the sorter was created without index tracking")
+    specs = [(keys_in, "keys_in", key_code, n, False), (keys_out, "keys_out", key_code, n, False)]
+    if idx_in is not None:
+        specs += [(idx_in, "idx_in", None, n, True), (idx_out, "idx_out", None, n, True)]
+    _check_tensors(specs)
+
+
+def validate_offsets(offsets):
+    """segment offsets: 1-D int64, starting at >= 0, non-decreasing; -> array"""
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    if off.ndim != 1 or off.size < 1:
+        raise ValueError("offsets must be a 1-D array of num_segments + 1 entries")
+    if off[0] < 0 or (off.size > 1 and np.any(off[1:] < off[:-1])):
+        raise ValueError("offsets must start at >= 0 and be non-decreasing")
+    return off
+
+
+def validate_multisort_args(key_code, arr_size, keys_in, keys_out, idx_in=None, idx_out=None):
+    """checks of a device MultiSort; returns the number of arrays"""
+    if int(arr_size) <= 0:
+        raise ValueError("arr_size must be positive")
+    n = keys_in.numel()
+    if n % int(arr_size):
+        raise ValueError(f"keys_in holds {n} keys, not a multiple of arr_size {arr_size}")
+    specs = [(keys_in, "keys_in", key_code, n, False), (keys_out, "keys_out", key_code, n, False)]
+    if idx_in is not None:
+        specs.append((idx_in, "idx_in", None, n, True))
+    if idx_out is not None:
+        specs.append((idx_out, "idx_out", None, n, True))
+    _check_tensors(specs)
+    return n // int(arr_size)
+
+
 def _stream(stream):
     """hipStream_t to run on: the given one, else torch's current stream, so
     that device work is ordered after the torch ops that produced the inputs"""
@@ -77,11 +142,15 @@ class TurtleSort:
                                            int(bool(compress)), ctypes.byref(h)))
         self._h = h
         self._pending = None
+        self._graphs = weakref.WeakSet()
 
     # ---- lifetime --------------------------------------------------------------
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
-            self._lib.turtle_destroy(self._h)
+            # graphs use the handle's buffers: they go first
+            for g in list(getattr(self, "_graphs", ())):
+                g.close()
+            _lib.check(self._lib.turtle_destroy(self._h))
             self._h = None
 
     def __del__(self):
@@ -156,35 +225,50 @@ class TurtleSort:
     # ---- device-resident forms (torch tensors on the GPU) --------------------------
     def sort_device(self, keys_in, keys_out, idx_in=None, idx_out=None, stream=None):
         n = keys_in.numel()
+        validate_sort_args(self.dtype, self.track_index, n, keys_in, keys_out, idx_in, idx_out)
         _lib.check(self._lib.turtle_sort_device(
             self._h, _ptr(keys_in), _ptr(idx_in), _ptr(keys_out), _ptr(idx_out), n, _stream(stream)))
 
     def segmented_sort_device(self, keys_in, keys_out, offsets, idx_in=None, idx_out=None, stream=None):
-        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        off = validate_offsets(offsets)
+        validate_sort_args(self.dtype, self.track_index, int(off[-1]), keys_in, keys_out, idx_in, idx_out)
         _lib.check(self._lib.turtle_segmented_sort_device(
             self._h, _ptr(keys_in), _ptr(idx_in), _ptr(keys_out), _ptr(idx_out),
             off.ctypes.data_as(ctypes.c_void_p), off.size - 1, _stream(stream)))
 
     def segmented_graph(self, keys_in, keys_out, offsets, idx_in=None, idx_out=None):
         """hipGraph of one segmented engine run over these buffers
-        (turtle_segmented_graph_create); replay with .launch()."""
-        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        (turtle_segmented_graph_create); replay with .launch().  The graph
+        keeps this sorter alive; close() of the sorter closes its graphs."""
+        off = validate_offsets(offsets)
+        validate_sort_args(self.dtype, self.track_index, int(off[-1]), keys_in, keys_out, idx_in, idx_out)
         g = ctypes.c_void_p()
         _lib.check(self._lib.turtle_segmented_graph_create(
             self._h, _ptr(keys_in), _ptr(idx_in), _ptr(keys_out), _ptr(idx_out),
             off.ctypes.data_as(ctypes.c_void_p), off.size - 1, ctypes.byref(g)))
-        return SegmentGraph(self._lib, g)
+        sg = SegmentGraph(self._lib, g, self)
+        self._graphs.add(sg)
+        return sg
 
     def multi_sort_device(self, arr_size, keys_in, keys_out, stream=None):
-        num = keys_in.numel() // arr_size
+        num = validate_multisort_args(self.dtype, arr_size, keys_in, keys_out)
         _lib.check(self._lib.turtle_multisort_device(
             self._h, self.dtype, int(arr_size), _ptr(keys_in), _ptr(keys_out), num, _stream(stream)))
 
     def multi_sort_index_device(self, arr_size, keys_in, keys_out, idx_out, idx_in=None, stream=None):
-        num = keys_in.numel() // arr_size
+        if idx_out is None:
+            raise ValueError("idx_out is required")
+        num = validate_multisort_args(self.dtype, arr_size, keys_in, keys_out, idx_in, idx_out)
         _lib.check(self._lib.turtle_multisort_index_device(
             self._h, self.dtype, int(arr_size), _ptr(keys_in), _ptr(idx_in), _ptr(keys_out), _ptr(idx_out), num,
             _stream(stream)))
+
+    def query(self):
+        """True while the sorter's last device work still runs (turtle_query)"""
+        r = self._lib.turtle_query(self._h)
+        if r < 0:
+            _lib.check(-r)
+        return bool(r)
 
     def synchronize(self):
         st = _lib.Stats()
@@ -234,9 +318,10 @@ class TurtleSort:
 class SegmentGraph:
     """A captured segmented sort; launch() replays it on a stream."""
 
-    def __init__(self, lib, g):
+    def __init__(self, lib, g, sorter):
         self._lib = lib
         self._g = g
+        self._sorter = sorter   # the graph replays into the sorter's buffers
 
     def launch(self, stream=None):
         _lib.check(self._lib.turtle_graph_launch(self._g, _stream(stream)))
@@ -245,6 +330,7 @@ class SegmentGraph:
         if getattr(self, "_g", None) is not None and self._g.value:
             self._lib.turtle_graph_destroy(self._g)
             self._g = None
+        self._sorter = None
 
     def __del__(self):
         try:

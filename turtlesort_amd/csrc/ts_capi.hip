@@ -147,6 +147,47 @@ struct EventTimer : ts::LaunchTimer {
 
 }  // namespace
 
+// Everything an engine run writes besides the key and index buffers: the
+// control block, segment, tile and LDS-task tables, level records and the
+// per-key bins.  A handle owns one set; every captured graph owns its own,
+// so that no later call on the handle frees or resizes what a graph's
+// kernel arguments point at.
+struct EngineTables {
+    ts::Ctrl *ctrl = nullptr;
+    ts::Seg *segs = nullptr;
+    uint32_t *tcnt = nullptr, *tpre = nullptr, *spart = nullptr;
+    uint32_t *tile_seg = nullptr;
+    ts::LdsTask *ldst = nullptr;
+    uint32_t *ldsq = nullptr;
+    ts::Level *levels = nullptr;
+    uint8_t *bins = nullptr;    // per-key bin of the current level (count -> scatter), per tile
+    uint32_t segcap = 0, tilecap = 0, ldscap = 0;
+    uint32_t bins_tiles = 0;    // tiles the bins buffer holds
+};
+
+// The engine run last queued on a handle.  An eager run is queued with the
+// estimated number of partition levels and returns; its level check (are
+// segments left? did a table overflow?) is resolved by turtle_synchronize or
+// by the next call on the handle, which queue more levels, or grow the
+// tables and run again from the untouched input, as needed.
+struct EngineRun {
+    bool engine = false;        // the handle's last device work was an engine run
+    bool pending = false;       // its level check is not resolved yet
+    bool graph = false;         // a graph replay (no resolution: its levels are fixed)
+    const EngineTables *tab = nullptr;
+    const void *kin = nullptr;  // depth-0 input (never written by the run)
+    const uint32_t *iin = nullptr;
+    void *kout = nullptr;
+    uint32_t *iout = nullptr;
+    uint64_t total = 0;
+    const uint32_t *dev_off = nullptr;
+    uint32_t nseg = 0;
+    hipStream_t st = nullptr;
+    uint32_t levels = 0;        // partition levels queued so far
+    uint32_t grows = 0;         // table growths of this run
+    turtle_tuning tune{};       // tuning the run was queued with
+};
+
 struct turtle_sorter {
     int dtype = 0, kb = 0, kt = 0;   // key type, key bytes, kernel key code
     uint64_t flip = 0;
@@ -156,22 +197,12 @@ struct turtle_sorter {
     int cus = 0;
     hipStream_t stream = nullptr;
     hipStream_t last_stream = nullptr;
-    // key / index buffers of max_n elements
+    // key / index buffers of max_n elements; kin / iin also keep an
+    // in-place device call's input for a rerun
     void *kin = nullptr, *kout = nullptr, *kt0 = nullptr, *kt1 = nullptr;
     uint32_t *iin = nullptr, *iout = nullptr, *it0 = nullptr, *it1 = nullptr;
-    uint8_t *bins = nullptr;   // per-key bin of the current level (count -> scatter), per tile
-    uint32_t bins_tiles = 0;   // tiles the bins buffer holds
-    // engine tables
-    ts::Ctrl *ctrl = nullptr;
+    EngineTables tab;                // the handle's engine tables
     ts::Ctrl *ctrl_host = nullptr;   // pinned readback copy
-    ts::Seg *segs = nullptr;
-    uint32_t *tcnt = nullptr, *tpre = nullptr, *spart = nullptr;
-    uint32_t *tile_seg = nullptr;
-    ts::LdsTask *ldst = nullptr;
-    uint32_t *ldsq = nullptr;
-    ts::Level *levels = nullptr;
-    ts::Level *levels_host = nullptr;  // pinned readback (a Level followed by a Ctrl)
-    uint32_t segcap = 0, tilecap = 0, ldscap = 0;
     uint32_t *dev_off = nullptr;
     size_t dev_off_cap = 0;
     std::vector<uint32_t> host_off;
@@ -180,8 +211,12 @@ struct turtle_sorter {
     bool tune_fixed = false;     // set by turtle_set_tuning: the tuning table is not consulted
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t lv_ev = nullptr;   // end of a batch of levels (the read-back waits for it on stream2)
+    hipEvent_t rb_ev = nullptr;   // end of the read-back of a batch's level state
+    ts::Level *levels_host = nullptr;  // pinned readback (a Level followed by a Ctrl)
     bool timed = false;
     uint64_t last_n = 0;
+    EngineRun run;               // the last engine run
+    int live_graphs = 0;         // graphs captured on this handle and not destroyed
     // StartSorting / Sync state
     void *host_keys = nullptr;
     int32_t *host_idx = nullptr;
@@ -194,50 +229,39 @@ struct turtle_sorter {
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
     bool stage_used[2] = {false, false};
     ts::CopyPool *pool = nullptr;        // threads of the user <-> staging copies
-    hipStream_t stream2 = nullptr;       // second stream of the chunked host MultiSort
-    // what the last engine run needs to be run again after its tables grow
-    bool last_engine = false;
-    uint64_t last_total = 0;
-    uint32_t last_nroots = 0;
-    uint32_t levels_run = 0;
+    hipStream_t stream2 = nullptr;       // read-backs; second stream of the chunked host MultiSort
     // per-kernel launch timing (turtle_set_profiling)
     EventTimer *timer = nullptr;
 };
 
 namespace {
 
-void free_tables(turtle_sorter *h)
+void free_tables(EngineTables &t)
 {
-    void *bufs[] = {h->segs, h->tcnt, h->tpre, h->spart, h->tile_seg, h->ldst, h->ldsq, h->levels, h->bins};
+    void *bufs[] = {t.ctrl, t.segs, t.tcnt, t.tpre, t.spart, t.tile_seg, t.ldst, t.ldsq, t.levels, t.bins};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
-    h->segs = nullptr;
-    h->bins = nullptr;
-    h->bins_tiles = 0;
-    h->tcnt = h->tpre = h->spart = nullptr;
-    h->tile_seg = nullptr;
-    h->ldst = nullptr;
-    h->ldsq = nullptr;
-    h->levels = nullptr;
-    h->segcap = h->tilecap = h->ldscap = 0;
+    t = EngineTables{};
 }
 
-// Sizes the segment, tile and LDS-task tables for a run over `total` keys
-// in `nroots` root segments.  Segments longer than the cutoff are disjoint
-// within a level, so a level holds fewer than total/cutoff of them; with
-// sample-quantile pivots each level shrinks them about S+1 times, so all
-// levels together stay near 1.4 total/cutoff (8x that is allocated).  Tiles
-// are per level: at most total/TILE + (segments of the level).  Every
-// segment yields at most S+1 LDS tasks.  `scale` grows the tables after an
-// overflow.  TURTLE_SEGCAP overrides the segment capacity (tests force the
-// overflow path with it).
-int ensure_tables(turtle_sorter *h, uint64_t total, uint32_t nroots, uint32_t scale)
+// Sizes engine tables for a run over `total` keys in `nroots` root segments
+// (the control block is allocated and cleared once).  Segments longer than
+// the cutoff are disjoint within a level, so a level holds fewer than
+// total/cutoff of them; with sample-quantile pivots each level shrinks them
+// about S+1 times, so all levels together stay near 1.4 total/cutoff (8x
+// that is allocated).  Tiles are per level: at most total/TILE + (segments
+// of the level).  Every segment yields at most S+1 LDS tasks.  `scale` > 1
+// grows the tables after an overflow.  TURTLE_SEGCAP replaces the segment
+// capacity of the first sizing (scale 1) only, so that tests can force an
+// overflow and the growth that follows it.
+int ensure_tables(const turtle_sorter *h, const turtle_tuning &tu, EngineTables &t, uint64_t total, uint32_t nroots,
+                  uint32_t scale)
 {
-    const uint64_t cutoff = (uint64_t)h->tune.lds_cutoff;
+    const uint64_t cutoff = (uint64_t)tu.lds_cutoff;
     const uint64_t tile = ts::engine_tile(h->kb);
-    const uint64_t S = (uint64_t)h->tune.splitters;
+    const uint64_t S = (uint64_t)tu.splitters;
     uint64_t segcap = (8 * (total / cutoff + 1) + 1024) * scale + nroots;
-    const uint64_t force = env_u64("TURTLE_SEGCAP", 0);
+    const uint64_t force = scale == 1 ? env_u64("TURTLE_SEGCAP", 0) : 0;
     if (force) segcap = force;
     const uint64_t tilecap = (total / tile + 1) + segcap;
     const uint64_t ldscap = segcap * (S + 1) + nroots + 1024;
@@ -249,30 +273,37 @@ int ensure_tables(turtle_sorter *h, uint64_t total, uint32_t nroots, uint32_t sc
     if (bins_tiles > tilecap) bins_tiles = tilecap;
     if (segcap > 0xffffff00ull || tilecap > 0xffffff00ull || ldscap > 0xffffff00ull)
         return fail(TURTLE_ERR_CAPACITY, "work tables would exceed 2^32 entries");
-    if (h->segs && h->segcap >= segcap && h->tilecap >= tilecap && h->ldscap >= ldscap &&
-        h->bins_tiles >= bins_tiles && (!force || h->segcap == segcap))
+    if (!t.ctrl) {
+        HIPCHK(hipMalloc((void **)&t.ctrl, sizeof(ts::Ctrl)), "hipMalloc(control block)");
+        HIPCHK(hipMemset(t.ctrl, 0, sizeof(ts::Ctrl)), "hipMemset(control block)");
+    }
+    if (t.segs && t.segcap >= segcap && t.tilecap >= tilecap && t.ldscap >= ldscap && t.bins_tiles >= bins_tiles)
         return TURTLE_OK;
     // the old tables may still be in use by a run in flight
-    if (h->segs) HIPCHK(hipDeviceSynchronize(), "hipDeviceSynchronize");
-    free_tables(h);
-    HIPCHK(hipMalloc((void **)&h->segs, segcap * sizeof(ts::Seg)), "hipMalloc(segments)");
+    if (t.segs) HIPCHK(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    ts::Ctrl *ctrl = t.ctrl;
+    t.ctrl = nullptr;
+    free_tables(t);
+    t.ctrl = ctrl;
+    HIPCHK(hipMalloc((void **)&t.segs, segcap * sizeof(ts::Seg)), "hipMalloc(segments)");
     const uint64_t entries = tilecap * ts::MAX_BINS;
-    HIPCHK(hipMalloc((void **)&h->tcnt, entries * sizeof(uint32_t)), "hipMalloc(tile counts)");
-    HIPCHK(hipMalloc((void **)&h->tpre, entries * sizeof(uint32_t)), "hipMalloc(tile prefixes)");
-    HIPCHK(hipMalloc((void **)&h->spart, (entries / ts::SCAN_CHUNK + 2) * sizeof(uint32_t)), "hipMalloc(scan sums)");
-    HIPCHK(hipMalloc((void **)&h->tile_seg, tilecap * sizeof(uint32_t)), "hipMalloc(tile map)");
-    HIPCHK(hipMalloc((void **)&h->ldst, ldscap * sizeof(ts::LdsTask)), "hipMalloc(lds tasks)");
-    HIPCHK(hipMalloc((void **)&h->ldsq, ldscap * sizeof(uint32_t)), "hipMalloc(lds order)");
-    HIPCHK(hipMalloc((void **)&h->levels, ts::MAX_LEVELS * sizeof(ts::Level)), "hipMalloc(levels)");
-    HIPCHK(hipMalloc((void **)&h->bins, bins_tiles * tile), "hipMalloc(bins)");
-    h->bins_tiles = (uint32_t)bins_tiles;
-    h->segcap = (uint32_t)segcap;
-    h->tilecap = (uint32_t)tilecap;
-    h->ldscap = (uint32_t)ldscap;
+    HIPCHK(hipMalloc((void **)&t.tcnt, entries * sizeof(uint32_t)), "hipMalloc(tile counts)");
+    HIPCHK(hipMalloc((void **)&t.tpre, entries * sizeof(uint32_t)), "hipMalloc(tile prefixes)");
+    HIPCHK(hipMalloc((void **)&t.spart, (entries / ts::SCAN_CHUNK + 2) * sizeof(uint32_t)), "hipMalloc(scan sums)");
+    HIPCHK(hipMalloc((void **)&t.tile_seg, tilecap * sizeof(uint32_t)), "hipMalloc(tile map)");
+    HIPCHK(hipMalloc((void **)&t.ldst, ldscap * sizeof(ts::LdsTask)), "hipMalloc(lds tasks)");
+    HIPCHK(hipMalloc((void **)&t.ldsq, ldscap * sizeof(uint32_t)), "hipMalloc(lds order)");
+    HIPCHK(hipMalloc((void **)&t.levels, ts::MAX_LEVELS * sizeof(ts::Level)), "hipMalloc(levels)");
+    HIPCHK(hipMalloc((void **)&t.bins, bins_tiles * tile), "hipMalloc(bins)");
+    t.bins_tiles = (uint32_t)bins_tiles;
+    t.segcap = (uint32_t)segcap;
+    t.tilecap = (uint32_t)tilecap;
+    t.ldscap = (uint32_t)ldscap;
     return TURTLE_OK;
 }
 
-ts::Params make_params(turtle_sorter *h, const void *kin, const uint32_t *iin, void *kout, uint32_t *iout)
+ts::Params make_params(const turtle_sorter *h, const turtle_tuning &tu, const EngineTables &t, const void *kin,
+                       const uint32_t *iin, void *kout, uint32_t *iout)
 {
     ts::Params p{};
     p.kin = kin;
@@ -283,23 +314,23 @@ ts::Params make_params(turtle_sorter *h, const void *kin, const uint32_t *iin, v
     p.kt1 = h->kt1;
     p.it0 = h->it0;
     p.it1 = h->it1;
-    p.ctrl = h->ctrl;
-    p.segs = h->segs;
-    p.tcnt = h->tcnt;
-    p.tpre = h->tpre;
-    p.spart = h->spart;
-    p.tile_seg = h->tile_seg;
-    p.ldst = h->ldst;
-    p.ldsq = h->ldsq;
-    p.bins = h->bins;
-    p.levels = h->levels;
-    p.segcap = h->segcap;
-    p.tilecap = h->tilecap;
-    p.ldscap = h->ldscap;
-    p.bins_tiles = h->bins_tiles;
-    p.cutoff = (uint32_t)h->tune.lds_cutoff;
-    p.tiles_per_wg = (uint32_t)(h->tune.tiles_per_task > 0 ? h->tune.tiles_per_task : 1);
-    p.n_total = 0;   // set by run_engine
+    p.ctrl = t.ctrl;
+    p.segs = t.segs;
+    p.tcnt = t.tcnt;
+    p.tpre = t.tpre;
+    p.spart = t.spart;
+    p.tile_seg = t.tile_seg;
+    p.ldst = t.ldst;
+    p.ldsq = t.ldsq;
+    p.bins = t.bins;
+    p.levels = t.levels;
+    p.segcap = t.segcap;
+    p.tilecap = t.tilecap;
+    p.ldscap = t.ldscap;
+    p.bins_tiles = t.bins_tiles;
+    p.cutoff = (uint32_t)tu.lds_cutoff;
+    p.tiles_per_wg = (uint32_t)(tu.tiles_per_task > 0 ? tu.tiles_per_task : 1);
+    p.n_total = 0;   // set by the caller
     p.flip = h->flip;
     return p;
 }
@@ -323,7 +354,9 @@ int ensure_readback(turtle_sorter *h)
     if (!h->levels_host)
         HIPCHK(hipHostMalloc((void **)&h->levels_host, sizeof(ts::Level) + sizeof(ts::Ctrl), hipHostMallocDefault),
                "hipHostMalloc");
+    if (!h->ctrl_host) HIPCHK(hipHostMalloc((void **)&h->ctrl_host, sizeof(ts::Ctrl), hipHostMallocDefault), "hipHostMalloc");
     if (!h->lv_ev) HIPCHK(hipEventCreateWithFlags(&h->lv_ev, hipEventDisableTiming), "hipEventCreate");
+    if (!h->rb_ev) HIPCHK(hipEventCreateWithFlags(&h->rb_ev, hipEventDisableTiming), "hipEventCreate");
     if (!h->stream2) HIPCHK(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking), "hipStreamCreate");
     return TURTLE_OK;
 }
@@ -339,13 +372,6 @@ hipStream_t stream_of(turtle_sorter *h, void *stream)
 
 turtle_tuning tuning_for(const turtle_sorter *h, bool idx, uint64_t total);
 
-// One engine run over device buffers.  In-place requests (kin == kout) first
-// copy the input into the second ping-pong buffer, which depth 0 only reads.
-// The levels are enqueued in batches; after a batch one small pinned
-// readback tells whether segments are left (none for the usual inputs after
-// the first batch).  A graph capture cannot read back, so it captures a
-// generous fixed number of levels and turtle_synchronize reports ERR_DEPTH
-// if that was not enough.
 // longest root of a run: the segments' longest (h->host_off holds the
 // offsets of a segmented run), else the whole input
 uint64_t longest_root(const turtle_sorter *h, bool segmented, uint64_t total)
@@ -359,9 +385,103 @@ uint64_t longest_root(const turtle_sorter *h, bool segmented, uint64_t total)
     return longest;
 }
 
-int run_engine(turtle_sorter *h, const void *kin, const uint32_t *iin, void *kout, uint32_t *iout, uint64_t total,
-               const uint32_t *dev_off, uint32_t nseg, hipStream_t st, bool capturing = false)
+// Queues one batch of run r on its stream: (first) reset + roots, then the
+// partition levels [r.levels, r.levels + nlev), then the LDS phase over the
+// whole task list (a later batch first clears its size classes: a task's
+// input range is never written by deeper levels, so sorting it again writes
+// the same output; its bytes and task counts are counted twice in the
+// statistics).  Behind the levels, the second stream reads the last level's
+// record and the control block back into pinned memory; the batch is then
+// pending until resolve() has looked at them.
+int queue_batch(turtle_sorter *h, EngineRun &r, const turtle_tuning &tu, uint32_t nlev, bool first)
 {
+    ts::Params p = make_params(h, tu, *r.tab, r.kin, r.iin, r.kout, r.iout);
+    p.n_total = (uint32_t)r.total;
+    const bool idx = r.iin != nullptr;
+    const int S = tu.splitters, wpc = tu.workgroups_per_cu;
+    if (r.levels + nlev > (uint32_t)ts::MAX_LEVELS) nlev = ts::MAX_LEVELS - r.levels;
+    if (!first) HIPCHK(ts::engine_clear_classes(p, r.st), "engine launch");
+    HIPCHK(ts::engine_levels(p, h->kt, idx, S, h->cus, wpc, r.dev_off, r.nseg, (uint32_t)r.total, r.levels, nlev, first,
+                             false, r.st, h->timer),
+           "engine launch");
+    r.levels += nlev;
+    if (r.levels > 0) HIPCHK(hipEventRecord(h->lv_ev, r.st), "hipEventRecord");
+    HIPCHK(ts::engine_levels(p, h->kt, idx, S, h->cus, wpc, r.dev_off, r.nseg, (uint32_t)r.total, r.levels, 0, false,
+                             true, r.st, h->timer),
+           "engine launch");
+    HIPCHK(hipEventRecord(h->ev1, r.st), "hipEventRecord");
+    if (r.levels > 0) {
+        ts::Ctrl *ch = (ts::Ctrl *)(h->levels_host + 1);
+        HIPCHK(hipStreamWaitEvent(h->stream2, h->lv_ev, 0), "hipStreamWaitEvent");
+        HIPCHK(hipMemcpyAsync(h->levels_host, r.tab->levels + (r.levels - 1), sizeof(ts::Level), hipMemcpyDeviceToHost,
+                              h->stream2),
+               "D2H(level)");
+        HIPCHK(hipMemcpyAsync(ch, r.tab->ctrl, sizeof(ts::Ctrl), hipMemcpyDeviceToHost, h->stream2), "D2H(ctrl)");
+        HIPCHK(hipEventRecord(h->rb_ev, h->stream2), "hipEventRecord");
+        r.pending = true;
+    }
+    return TURTLE_OK;
+}
+
+// TURTLE_LEVEL_BATCH caps the levels per batch (tests force the read-back
+// path with it)
+uint32_t level_batch_cap() { return (uint32_t)env_u64("TURTLE_LEVEL_BATCH", 0); }
+
+constexpr uint32_t MAX_GROWS = 3;
+
+// Resolves the pending level check of the handle's last engine run: waits
+// for the read-back of its last batch (not for its LDS phase), and queues
+// 4 more levels and the LDS phase again while segments are left; after a
+// table overflow it grows the tables (4x, 16x, 64x) and queues the whole
+// run again from its input, which no level writes.
+int resolve(turtle_sorter *h)
+{
+    EngineRun &r = h->run;
+    while (r.pending) {
+        r.pending = false;
+        HIPCHK(hipEventSynchronize(h->rb_ev), "hipEventSynchronize(read-back)");
+        const ts::Ctrl *c = (const ts::Ctrl *)(h->levels_host + 1);
+        const uint32_t left = c->seg_alloc > h->levels_host->se ? c->seg_alloc - h->levels_host->se : 0u;
+        const uint32_t err = c->error;
+        constexpr uint32_t OVERFLOW = ts::ERR_SEGS | ts::ERR_TILES | ts::ERR_LDSQ;
+        if ((err & OVERFLOW) && !(err & ~OVERFLOW) && r.grows < MAX_GROWS && r.tab == &h->tab) {
+            r.grows++;
+            HIPCHK(hipStreamSynchronize(r.st), "hipStreamSynchronize");
+            int rc = ensure_tables(h, r.tune, h->tab, r.total, r.nseg, 1u << (2 * r.grows));
+            if (rc) return rc;
+            r.levels = 0;
+            const uint32_t cap = level_batch_cap();
+            uint32_t nlev = levels_estimate(longest_root(h, r.dev_off != nullptr, r.total), (uint64_t)r.tune.lds_cutoff,
+                                            (uint64_t)r.tune.splitters);
+            if (cap && nlev > cap) nlev = cap;
+            HIPCHK(hipEventRecord(h->ev0, r.st), "hipEventRecord");
+            if ((rc = queue_batch(h, r, r.tune, nlev, true))) return rc;
+            continue;
+        }
+        if (left && !err && r.levels < (uint32_t)ts::MAX_LEVELS) {
+            const uint32_t cap = level_batch_cap();
+            const int rc = queue_batch(h, r, r.tune, cap && cap < 4 ? cap : 4, false);
+            if (rc) return rc;
+        }
+    }
+    return TURTLE_OK;
+}
+
+// One engine run over device buffers.  In-place requests (kin == kout)
+// first copy the input into the handle's input buffer, which no level
+// writes, so that a rerun after a table overflow still has it.  Eager runs
+// resolve the previous run first, queue the estimated levels and the LDS
+// phase, and return (see resolve).  capture != nullptr: recorded into a
+// graph, with the graph's own tables and a fixed level count (returned in
+// *captured_levels): the estimate plus two spare levels, or
+// $TURTLE_GRAPH_LEVELS (tests force a too-shallow graph with it).
+int run_engine(turtle_sorter *h, const void *kin, const uint32_t *iin, void *kout, uint32_t *iout, uint64_t total,
+               const uint32_t *dev_off, uint32_t nseg, hipStream_t st, EngineTables *capture = nullptr,
+               uint32_t *captured_levels = nullptr)
+{
+    const bool capturing = capture != nullptr;
+    int rc = TURTLE_OK;
+    if (!capturing && (rc = resolve(h))) return rc;
     const bool idx = iin != nullptr && iout != nullptr;
     // the longest root decides the level count, and the tuning: the segments
     // of a batch are partitioned independently, so a batch is tuned like one
@@ -369,108 +489,83 @@ int run_engine(turtle_sorter *h, const void *kin, const uint32_t *iin, void *kou
     // resolved and sized the tables for)
     const uint64_t longest = longest_root(h, dev_off != nullptr, total);
     if (!capturing) h->tune = tuning_for(h, idx, longest);
-    int rc = ensure_tables(h, total, nseg, 1);
-    if (rc) return rc;
+    EngineTables &tab = capturing ? *capture : h->tab;
+    if ((rc = ensure_tables(h, h->tune, tab, total, nseg, 1))) return rc;
     if ((rc = ensure_readback(h))) return rc;
     if (total > 0 && kin == kout) {
-        HIPCHK(hipMemcpyAsync(h->kt1, kin, total * h->kb, hipMemcpyDeviceToDevice, st), "hipMemcpyAsync(in-place keys)");
-        kin = h->kt1;
+        HIPCHK(hipMemcpyAsync(h->kin, kin, total * h->kb, hipMemcpyDeviceToDevice, st), "hipMemcpyAsync(in-place keys)");
+        kin = h->kin;
     }
     if (idx && total > 0 && (const void *)iin == (const void *)iout) {
-        HIPCHK(hipMemcpyAsync(h->it1, iin, total * 4, hipMemcpyDeviceToDevice, st), "hipMemcpyAsync(in-place idx)");
-        iin = h->it1;
+        HIPCHK(hipMemcpyAsync(h->iin, iin, total * 4, hipMemcpyDeviceToDevice, st), "hipMemcpyAsync(in-place idx)");
+        iin = h->iin;
     }
-    ts::Params p = make_params(h, kin, idx ? iin : nullptr, kout, idx ? iout : nullptr);
-    p.n_total = (uint32_t)total;
-    const int S = h->tune.splitters, wpc = h->tune.workgroups_per_cu;
-    uint32_t nlev = levels_estimate(longest, (uint64_t)h->tune.lds_cutoff, (uint64_t)S);
-    if (!capturing) HIPCHK(hipEventRecord(h->ev0, st), "hipEventRecord");
+    EngineRun r;
+    r.engine = true;
+    r.tab = &tab;
+    r.kin = kin;
+    r.iin = idx ? iin : nullptr;
+    r.kout = kout;
+    r.iout = idx ? iout : nullptr;
+    r.total = total;
+    r.dev_off = dev_off;
+    r.nseg = nseg;
+    r.st = st;
+    r.tune = h->tune;
+    uint32_t nlev = levels_estimate(longest, (uint64_t)h->tune.lds_cutoff, (uint64_t)h->tune.splitters);
     if (capturing) {
-        nlev += 2;   // no read-back inside a graph: two spare levels
+        // no read-back inside a graph: two spare levels
+        const uint64_t force = env_u64("TURTLE_GRAPH_LEVELS", 0);
+        nlev = force ? (uint32_t)force : nlev + 2;
         if (nlev > (uint32_t)ts::MAX_LEVELS) nlev = ts::MAX_LEVELS;
-        HIPCHK(ts::engine_levels(p, h->kt, idx, S, h->cus, wpc, dev_off, nseg, (uint32_t)total, 0, nlev, true, true, st),
+        ts::Params p = make_params(h, h->tune, tab, kin, r.iin, kout, r.iout);
+        p.n_total = (uint32_t)total;
+        HIPCHK(ts::engine_levels(p, h->kt, idx, h->tune.splitters, h->cus, h->tune.workgroups_per_cu, dev_off, nseg,
+                                 (uint32_t)total, 0, nlev, true, true, st),
                "engine launch");
-        h->levels_run = nlev;
-    } else if (nlev == 0) {
-        // every root fits the LDS sort: no partition level, no read-back
-        HIPCHK(ts::engine_levels(p, h->kt, idx, S, h->cus, wpc, dev_off, nseg, (uint32_t)total, 0, 0, true, true, st,
-                                 h->timer),
-               "engine launch");
-        h->levels_run = 0;
-    } else {
-        // The LDS phase is queued right behind the estimated levels, and the
-        // read-back of the level state runs on the second stream as soon as
-        // the levels are done, while the LDS phase runs: the device never
-        // waits for the host, and the call returns while the LDS phase is
-        // still running.  When the read-back shows segments left, more
-        // levels run and then the LDS phase again over the whole task list:
-        // a task's input range is never written by deeper levels, so sorting
-        // it again writes the same output (its bytes and task counts are
-        // counted twice in the statistics).
-        // TURTLE_LEVEL_BATCH caps the levels per batch (tests force the
-        // read-back path with it)
-        const uint32_t batch_cap = (uint32_t)env_u64("TURTLE_LEVEL_BATCH", 0);
-        if (batch_cap && nlev > batch_cap) nlev = batch_cap;
-        uint32_t L0 = 0;
-        bool first = true;
-        for (;;) {
-            if (L0 + nlev > (uint32_t)ts::MAX_LEVELS) nlev = ts::MAX_LEVELS - L0;
-            if (!first) HIPCHK(ts::engine_clear_classes(p, st), "engine launch");
-            HIPCHK(ts::engine_levels(p, h->kt, idx, S, h->cus, wpc, dev_off, nseg, (uint32_t)total, L0, nlev, first,
-                                     false, st, h->timer),
-                   "engine launch");
-            HIPCHK(hipEventRecord(h->lv_ev, st), "hipEventRecord");
-            HIPCHK(ts::engine_levels(p, h->kt, idx, S, h->cus, wpc, dev_off, nseg, (uint32_t)total, L0 + nlev, 0, false,
-                                     true, st, h->timer),
-                   "engine launch");
-            first = false;
-            L0 += nlev;
-            // segments created by the last level are the ones still to do
-            // (seg_alloc and the level record do not change in the LDS phase)
-            ts::Ctrl *ch = (ts::Ctrl *)(h->levels_host + 1);
-            HIPCHK(hipStreamWaitEvent(h->stream2, h->lv_ev, 0), "hipStreamWaitEvent");
-            HIPCHK(hipMemcpyAsync(h->levels_host, h->levels + (L0 - 1), sizeof(ts::Level), hipMemcpyDeviceToHost,
-                                  h->stream2),
-                   "D2H(level)");
-            HIPCHK(hipMemcpyAsync(ch, h->ctrl, sizeof(ts::Ctrl), hipMemcpyDeviceToHost, h->stream2), "D2H(ctrl)");
-            HIPCHK(hipStreamSynchronize(h->stream2), "hipStreamSynchronize");
-            const uint32_t left = ch->seg_alloc > h->levels_host->se ? ch->seg_alloc - h->levels_host->se : 0u;
-            if (left == 0 || ch->error != 0 || L0 >= (uint32_t)ts::MAX_LEVELS) break;
-            nlev = batch_cap && batch_cap < 4 ? batch_cap : 4;
-        }
-        h->levels_run = L0;
+        if (captured_levels) *captured_levels = nlev;
+        return TURTLE_OK;
     }
-    if (!capturing) HIPCHK(hipEventRecord(h->ev1, st), "hipEventRecord");
-    h->timed = !capturing;
+    const uint32_t cap = level_batch_cap();
+    if (cap && nlev > cap) nlev = cap;
+    HIPCHK(hipEventRecord(h->ev0, st), "hipEventRecord");
+    if ((rc = queue_batch(h, r, h->tune, nlev, true))) return rc;
+    h->run = r;
+    h->timed = true;
     h->last_stream = st;
     h->last_n = total;
-    h->last_engine = true;
-    h->last_total = total;
-    h->last_nroots = nseg;
     return TURTLE_OK;
 }
 
+// Reads the control block (and the last level's record) of the handle's
+// last engine run after its stream is done.
 int read_stats(turtle_sorter *h, turtle_stats *st, uint32_t *err_out)
 {
-    if (!h->ctrl_host) HIPCHK(hipHostMalloc((void **)&h->ctrl_host, sizeof(ts::Ctrl), hipHostMallocDefault), "hipHostMalloc");
     int rc = ensure_readback(h);
     if (rc) return rc;
-    HIPCHK(hipMemcpyAsync(h->ctrl_host, h->ctrl, sizeof(ts::Ctrl), hipMemcpyDeviceToHost, h->last_stream), "D2H(ctrl)");
-    if (h->last_engine && h->levels_run > 0)
-        HIPCHK(hipMemcpyAsync(h->levels_host, h->levels + (h->levels_run - 1), sizeof(ts::Level), hipMemcpyDeviceToHost,
-                              h->last_stream), "D2H(level)");
-    HIPCHK(hipStreamSynchronize(h->last_stream), "hipStreamSynchronize");
-    const ts::Ctrl c = *h->ctrl_host;
-    uint32_t err = h->last_engine ? c.error : 0u;
+    const EngineRun &r = h->run;
+    const bool eng = r.engine && r.tab && r.tab->ctrl;
+    ts::Ctrl c{};
+    if (eng) {
+        HIPCHK(hipMemcpyAsync(h->ctrl_host, r.tab->ctrl, sizeof(ts::Ctrl), hipMemcpyDeviceToHost, h->last_stream),
+               "D2H(ctrl)");
+        if (r.levels > 0)
+            HIPCHK(hipMemcpyAsync(h->levels_host, r.tab->levels + (r.levels - 1), sizeof(ts::Level),
+                                  hipMemcpyDeviceToHost, h->last_stream),
+                   "D2H(level)");
+        HIPCHK(hipStreamSynchronize(h->last_stream), "hipStreamSynchronize");
+        c = *h->ctrl_host;
+    }
+    uint32_t err = eng ? c.error : 0u;
     // segments created by the last level were never partitioned
-    if (h->last_engine && h->levels_run > 0 && c.seg_alloc > h->levels_host->se) err |= ts::ERR_DEPTH;
+    if (eng && r.levels > 0 && c.seg_alloc > h->levels_host->se) err |= ts::ERR_DEPTH;
     float ms = 0.f;
     if (h->timed) HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1), "hipEventElapsedTime");
     if (st) {
         memset(st, 0, sizeof(*st));
         st->device_ms = ms;
         st->n = h->last_n;
-        if (h->last_engine) {
+        if (eng) {
             st->bytes_count = c.bytes_count;
             st->bytes_scatter = c.bytes_scatter;
             st->bytes_lds = c.bytes_lds;
@@ -722,15 +817,13 @@ int turtle_create(int dtype, int64_t max_n, int track_index, int compress, turtl
             return rc;
         }
     }
-    if ((e = hipMalloc((void **)&h->ctrl, sizeof(ts::Ctrl))) != hipSuccess ||
-        (e = hipMemset(h->ctrl, 0, sizeof(ts::Ctrl))) != hipSuccess ||
-        (e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)) != hipSuccess ||
+    if ((e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipEventCreate(&h->ev0)) != hipSuccess || (e = hipEventCreate(&h->ev1)) != hipSuccess) {
         turtle_destroy(h);
         return fail_hip(e, "turtle_create");
     }
     h->last_stream = h->stream;
-    if ((rc = ensure_tables(h, (uint64_t)max_n, 1, 1))) {
+    if ((rc = ensure_tables(h, h->tune, h->tab, (uint64_t)max_n, 1, 1))) {
         turtle_destroy(h);
         return rc;
     }
@@ -745,15 +838,18 @@ int turtle_create(int dtype, int64_t max_n, int track_index, int compress, turtl
 int turtle_destroy(turtle_handle h)
 {
     if (!h) return TURTLE_OK;
+    if (h->live_graphs > 0)
+        return fail(TURTLE_ERR_STATE, "graphs captured on this handle are alive: destroy them first");
+    (void)resolve(h);
     if (h->last_stream != h->stream) (void)hipStreamSynchronize(h->last_stream);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     if (h->stream2) (void)hipStreamSynchronize(h->stream2);
     delete h->timer;
     delete h->pool;
-    void *bufs[] = {h->kin, h->kout, h->kt0, h->kt1, h->iin, h->iout, h->it0, h->it1, h->ctrl, h->dev_off};
+    void *bufs[] = {h->kin, h->kout, h->kt0, h->kt1, h->iin, h->iout, h->it0, h->it1, h->dev_off};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
-    free_tables(h);
+    free_tables(h->tab);
     if (h->ctrl_host) (void)hipHostFree(h->ctrl_host);
     if (h->levels_host) (void)hipHostFree(h->levels_host);
     for (int k = 0; k < 2; k++) {
@@ -762,6 +858,7 @@ int turtle_destroy(turtle_handle h)
     }
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->lv_ev) (void)hipEventDestroy(h->lv_ev);
+    if (h->rb_ev) (void)hipEventDestroy(h->rb_ev);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     if (h->stream2) (void)hipStreamDestroy(h->stream2);
@@ -932,7 +1029,10 @@ int turtle_segmented_sort_device(turtle_handle h, const void *keys_in, const int
     if (total > h->max_n) return fail(TURTLE_ERR_CAPACITY, "segments exceed the handle's max_n");
     if (total > 0 && (!keys_in || !keys_out)) return fail(TURTLE_ERR_INVALID, "NULL key buffer");
     hipStream_t st = stream_of(h, stream);
-    // the previous run's offsets may still be in use by the device
+    // the previous run's offsets may still be in use by the device (and by
+    // the levels its resolution adds)
+    int rc = resolve(h);
+    if (rc) return rc;
     HIPCHK(hipStreamSynchronize(h->last_stream), "hipStreamSynchronize");
     if (st != h->last_stream) HIPCHK(hipStreamSynchronize(st), "hipStreamSynchronize");
     const size_t need = (size_t)num_segments + 1;
@@ -970,6 +1070,8 @@ static int multisort_device(turtle_handle h, int dtype, int64_t arr_size, const 
         return fail(TURTLE_ERR_CAPACITY, "an indexed batch is limited to 2^32 keys");
     hipStream_t st = stream_of(h, stream);
     if ((uint64_t)arr_size <= ts::multisort_max_size(h->kt)) {
+        const int rc = resolve(h);   // the previous engine run's level check
+        if (rc) return rc;
         HIPCHK(hipEventRecord(h->ev0, st), "hipEventRecord");
         HIPCHK(ts::multisort_run(h->kt, h->flip, (uint32_t)arr_size, keys_in, keys_out, (const uint32_t *)idx_in,
                                  (uint32_t *)idx_out, (uint64_t)num_arrays, st),
@@ -978,7 +1080,7 @@ static int multisort_device(turtle_handle h, int dtype, int64_t arr_size, const 
         h->timed = true;
         h->last_stream = st;
         h->last_n = (uint64_t)arr_size * num_arrays;
-        h->last_engine = false;
+        h->run = EngineRun{};
         return TURTLE_OK;
     }
     // longer arrays: equal segments through the sort engine
@@ -1013,16 +1115,22 @@ int turtle_multisort_index_device(turtle_handle h, int dtype, int64_t arr_size, 
 int turtle_synchronize(turtle_handle h, turtle_stats *st)
 {
     if (!h) return fail(TURTLE_ERR_INVALID, "NULL handle");
+    int rc = resolve(h);   // more levels, or a rerun with grown tables, when needed
+    if (rc) return rc;
     HIPCHK(hipStreamSynchronize(h->last_stream), "hipStreamSynchronize");
     uint32_t err = 0;
-    int rc = read_stats(h, st, &err);
+    rc = read_stats(h, st, &err);
     if (rc) return rc;
-    rc = error_status(err);
-    if (rc == TURTLE_ERR_QUEUE_OVERFLOW) {
-        // grow the tables so that the next run fits
-        (void)ensure_tables(h, h->last_total, h->last_nroots, 4);
-    }
-    return rc;
+    return error_status(err);
+}
+
+int turtle_query(turtle_handle h)
+{
+    if (!h) return -fail(TURTLE_ERR_INVALID, "NULL handle");
+    const hipError_t e = hipStreamQuery(h->last_stream);
+    if (e == hipErrorNotReady) return 1;
+    if (e != hipSuccess) return -fail_hip(e, "hipStreamQuery");
+    return 0;
 }
 
 int turtle_sort_start(turtle_handle h, void *keys, int32_t *idx, int64_t n)
@@ -1034,10 +1142,13 @@ int turtle_sort_start(turtle_handle h, void *keys, int32_t *idx, int64_t n)
     if (idx && !h->track) return fail(TURTLE_ERR_INVALID, "handle created without index tracking");
     if (h->started) return fail(TURTLE_ERR_STATE, "StartSorting called twice without Sync");
     h->t_start = now_s();
+    // the previous run's level check, and its use of the input buffers
+    int rc = resolve(h);
+    if (rc) return rc;
+    if (h->last_stream != h->stream) HIPCHK(hipStreamSynchronize(h->last_stream), "hipStreamSynchronize");
     h->host_keys = keys;
     h->host_idx = idx;
     h->host_n = n;
-    int rc = TURTLE_OK;
     if (n > 0) {
         if ((rc = stage_h2d(h, h->kin, keys, (size_t)n * h->kb, h->stream))) return rc;
         if (idx && (rc = stage_h2d(h, h->iin, idx, (size_t)n * 4, h->stream))) return rc;
@@ -1054,15 +1165,9 @@ int turtle_sort_sync(turtle_handle h, double *seconds)
     if (!h) return fail(TURTLE_ERR_INVALID, "NULL handle");
     if (!h->started) return fail(TURTLE_ERR_STATE, "Sync without StartSorting");
     h->started = false;
+    // the level check of the run (more levels, or a rerun from the device
+    // input buffer with grown tables after an overflow), then the wait
     int rc = turtle_synchronize(h, nullptr);
-    if (rc == TURTLE_ERR_QUEUE_OVERFLOW) {
-        // the input is still intact in the device input buffer: run again
-        // with the grown tables
-        const bool idx = h->host_idx != nullptr;
-        rc = run_engine(h, h->kin, idx ? h->iin : nullptr, h->kout, idx ? h->iout : nullptr, (uint64_t)h->host_n,
-                        nullptr, 1, h->stream);
-        if (!rc) rc = turtle_synchronize(h, nullptr);
-    }
     if (rc) return rc;
     if (h->host_n > 0) {
         if ((rc = stage_d2h(h, h->host_keys, h->kout, (size_t)h->host_n * h->kb, h->stream))) return rc;
@@ -1093,8 +1198,9 @@ static int multisort_host(turtle_handle h, int64_t arr_size, int64_t num_arrays,
     const size_t kb = (size_t)h->kb, ib = idx ? 4u : 0u;
     const size_t kalign = 256;
     const size_t per_arr = (size_t)arr_size * (kb + ib) + kalign;
-    int rc = ensure_staging(h);
+    int rc = resolve(h);   // the previous engine run's level check
     if (rc) return rc;
+    if ((rc = ensure_staging(h))) return rc;
     if ((uint64_t)arr_size > ts::multisort_max_size(h->kt) || per_arr > STAGE_BYTES) {
         if ((rc = stage_h2d(h, h->kin, keys, (size_t)total * kb, h->stream))) return rc;
         if (idx && !local && (rc = stage_h2d(h, h->iin, idx, (size_t)total * 4, h->stream))) return rc;
@@ -1155,6 +1261,7 @@ static int multisort_host(turtle_handle h, int64_t arr_size, int64_t num_arrays,
     }
     h->timed = false;   // no single device interval to report
     h->last_stream = h->stream;
+    h->run = EngineRun{};
     return TURTLE_OK;
 }
 
@@ -1198,14 +1305,25 @@ struct turtle_graph {
     hipGraphExec_t exec = nullptr;
     uint64_t total = 0;
     uint32_t nseg = 0;
+    uint32_t nlev = 0;        // partition levels captured
+    EngineTables tab;         // the graph's own engine tables
+    bool counted = false;     // counted in h->live_graphs
 };
 
 int turtle_graph_destroy(turtle_graph_handle g)
 {
     if (!g) return TURTLE_OK;
+    turtle_handle h = g->h;
+    if (h && h->run.tab == &g->tab) {
+        // the handle's last run is a replay of this graph: let it finish
+        (void)hipStreamSynchronize(h->last_stream);
+        h->run = EngineRun{};
+    }
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
     if (g->dev_off) (void)hipFree(g->dev_off);
+    free_tables(g->tab);
+    if (h && g->counted) h->live_graphs--;
     delete g;
     return TURTLE_OK;
 }
@@ -1228,16 +1346,22 @@ int turtle_segmented_graph_create(turtle_handle h, const void *keys_in, const in
     if (total > h->max_n) return fail(TURTLE_ERR_CAPACITY, "segments exceed the handle's max_n");
     if (total > 0 && (!keys_in || !keys_out)) return fail(TURTLE_ERR_INVALID, "NULL key buffer");
     if (total > 0 && keys_in == keys_out) return fail(TURTLE_ERR_INVALID, "graph form needs keys_out != keys_in");
+    // the handle's previous run first (its level check may queue work)
+    int rc = resolve(h);
+    if (rc) return rc;
     turtle_graph *g = new turtle_graph();
     g->h = h;
     g->total = (uint64_t)total;
     g->nseg = (uint32_t)num_segments;
+    h->live_graphs++;
+    g->counted = true;
     std::vector<uint32_t> off((size_t)num_segments + 1);
     for (size_t s = 0; s < off.size(); s++) off[s] = (uint32_t)offsets[s];
     h->host_off = off;   // the level estimate reads the longest segment from here
-    // everything the run allocates is allocated before the capture starts
+    // everything the run allocates is allocated before the capture starts:
+    // the graph's own tables, sized for its tuning
     h->tune = tuning_for(h, idx_in != nullptr, longest_root(h, true, (uint64_t)total));
-    int rc = ensure_tables(h, (uint64_t)total, (uint32_t)num_segments, 1);
+    rc = ensure_tables(h, h->tune, g->tab, (uint64_t)total, (uint32_t)num_segments, 1);
     if (!rc) rc = ensure_readback(h);
     hipError_t e = hipSuccess;
     if (!rc) e = hipMalloc((void **)&g->dev_off, off.size() * sizeof(uint32_t));
@@ -1250,7 +1374,7 @@ int turtle_segmented_graph_create(turtle_handle h, const void *keys_in, const in
     if (!rc && e == hipSuccess) e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
     if (!rc && e == hipSuccess) {
         rc = run_engine(h, keys_in, (const uint32_t *)idx_in, keys_out, (uint32_t *)idx_out, (uint64_t)total,
-                        g->dev_off, (uint32_t)num_segments, cs, true);
+                        g->dev_off, (uint32_t)num_segments, cs, &g->tab, &g->nlev);
         hipGraph_t graph = nullptr;
         hipError_t e2 = hipStreamEndCapture(cs, &graph);
         g->graph = graph;
@@ -1269,14 +1393,23 @@ int turtle_segmented_graph_create(turtle_handle h, const void *keys_in, const in
 int turtle_graph_launch(turtle_graph_handle g, void *stream)
 {
     if (!g) return fail(TURTLE_ERR_INVALID, "NULL graph");
-    hipStream_t st = stream_of(g->h, stream);
+    turtle_handle h = g->h;
+    int rc = resolve(h);   // an eager run's level check queues nothing after the replay
+    if (rc) return rc;
+    hipStream_t st = stream_of(h, stream);
     HIPCHK(hipGraphLaunch(g->exec, st), "hipGraphLaunch");
-    g->h->last_stream = st;
-    g->h->last_n = g->total;
-    g->h->last_engine = true;
-    g->h->last_total = g->total;
-    g->h->last_nroots = g->nseg;
-    g->h->timed = false;
+    EngineRun r;
+    r.engine = true;
+    r.graph = true;
+    r.tab = &g->tab;
+    r.total = g->total;
+    r.nseg = g->nseg;
+    r.st = st;
+    r.levels = g->nlev;   // turtle_synchronize reads this level's record
+    h->run = r;
+    h->last_stream = st;
+    h->last_n = g->total;
+    h->timed = false;
     return TURTLE_OK;
 }
 

@@ -34,7 +34,11 @@ constexpr bool kt_float(int kt) { return (kt & KT_FLOAT) != 0; }
 
 constexpr int MAX_SPLIT = 31;
 constexpr int MAX_BINS = 2 * MAX_SPLIT + 1;
-constexpr int MAX_LEVELS = 64;
+// Depth bound: a segment whose child keeps more than half of its keys is
+// split by key range next (Seg::mode), which divides the range by S+1; so a
+// chain of levels holds at most log2(n / cutoff) halvings plus twice
+// ceil(key bits / log2(S+1)) range steps (85 for 64-bit keys at S = 3).
+constexpr int MAX_LEVELS = 128;
 constexpr int LDS_CLASSES = 20;        // LDS-sort size classes
 constexpr uint32_t SCAN_CHUNK = 4096;  // tile-count entries per workgroup of the level scan
 
@@ -73,9 +77,17 @@ struct LdsTask {
 // counts sit at tcnt[tile_base*NB + bin*ntiles + tile] (bin-major), so the
 // level-wide exclusive scan of tcnt gives, relative to the segment's first
 // entry, the destination of every (bin, tile) inside the segment.
+// Splitter choice of a segment (Seg::mode).
+enum : uint32_t {
+    SPLIT_SAMPLE = 0,   // sample quantiles
+    SPLIT_RANGE = 1     // S+1 equal slices of [lo, hi]: the parent's split left
+                        // this segment more than half of the parent's keys
+};
+
 struct Seg {
     uint32_t begin, len, depth, ntiles;
-    uint32_t tile_base, pad0, pad1, pad2;
+    uint32_t tile_base, mode, pad0, pad1;
+    uint64_t lo, hi;               // bounds of the segment's order values (inclusive)
     uint64_t split[MAX_SPLIT];     // splitters in order representation, ascending
     uint64_t pad3;
 };

@@ -63,6 +63,9 @@ __global__ void k_roots(Params p, const uint32_t *off, uint32_t nseg, uint32_t s
         sg->begin = beg;
         sg->len = len;
         sg->depth = 0;
+        sg->mode = SPLIT_SAMPLE;
+        sg->lo = 0;
+        sg->hi = ~0ull;   // the pivot clamps it to the key width
     } else if (small && lbase != ~0u) {
         LdsTask *t = &p.ldst[lbase + mask_rank(ms)];
         t->begin = beg;

@@ -680,7 +680,8 @@ struct LevelK {
         for (uint32_t s0 = lv.sb + blockIdx.x * CW; s0 < lv.se; s0 += gridDim.x * CW) {   // block-uniform
             const uint32_t sid = s0 + w;
             const uint32_t j = lane();
-            uint32_t cl = 0, cb = 0, depth = 0;
+            uint32_t cl = 0, cb = 0, depth = 0, cmode = SPLIT_SAMPLE;
+            uint64_t clo = 0, chi = 0;
             if (sid < lv.se) {   // wave-uniform
                 const Seg *sg = &p.segs[sid];
                 const uint32_t nt = sg->ntiles;
@@ -698,6 +699,14 @@ struct LevelK {
                         const uint32_t en = (b + 1 < (uint32_t)NB) ? p.tpre[e0 + (size_t)(b + 1) * nt] - p0 : len;
                         cl = en - st;
                         cb = begin + st;
+                        // bin 2j holds keys strictly between splitters j-1
+                        // and j (a non-empty bin has a key there, so the
+                        // bounds do not wrap); a child that kept more than
+                        // half of its parent is split by range next, which
+                        // bounds the depth by the key bits
+                        clo = j == 0 ? sg->lo : sg->split[j - 1] + 1;
+                        chi = j == (uint32_t)S ? sg->hi : sg->split[j] - 1;
+                        cmode = cl > len / 2 ? SPLIT_RANGE : SPLIT_SAMPLE;
                     }
                 }
             }
@@ -733,6 +742,9 @@ struct LevelK {
                     c->begin = cb;
                     c->len = cl;
                     c->depth = depth + 1;
+                    c->mode = cmode;
+                    c->lo = clo;
+                    c->hi = chi;
                 }
             } else if (small && lbase != ~0u) {
                 LdsTask *tk = &p.ldst[lbase + mask_rank(ms)];
@@ -1250,6 +1262,23 @@ struct LevelK {
                         sg->ntiles = 0;
                         sg->tile_base = 0;
                     }
+                } else if (sg->mode == SPLIT_RANGE) {
+                    // S+1 equal slices of the segment's bounds [lo, hi]: every
+                    // child's range is at most a (S+1)-th of this one's (a
+                    // range narrower than S+1 values makes every value above
+                    // lo a splitter, so the next level ends it), whatever the
+                    // keys' positions
+                    const U kmax = key_max<U>();
+                    const U lo = (U)sg->lo;
+                    const U hi = sg->hi > (uint64_t)kmax ? kmax : (U)sg->hi;
+                    const U span = hi > lo ? (U)(hi - lo) : (U)0;
+                    U wd = (U)(span / (U)(S + 1));
+                    if (wd == 0) wd = 1;
+                    if (l < (uint32_t)S) {
+                        const U off = (U)((U)(l + 1) * wd);
+                        sg->split[l] = (uint64_t)(off > span ? hi : (U)(lo + off));
+                    }
+                    ntiles = (len + TILE - 1) / TILE;
                 } else {
                     const uint32_t m = len < SAMPLE ? len : SAMPLE;
                     const U *src = src_keys<U>(p, depth) + begin;

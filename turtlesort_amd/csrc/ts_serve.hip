@@ -118,13 +118,9 @@ int run_batch(turtle_server *sv, Slot *s, std::vector<std::shared_ptr<Request>> 
         return TURTLE_ERR_DEVICE;
     int rc = turtle_segmented_sort_device(s->h, s->dkin, any_idx ? s->diin : nullptr, s->dkout,
                                           any_idx ? s->diout : nullptr, off.data(), (int64_t)batch.size(), s->stream);
+    // turtle_synchronize adds levels, or grows the tables and sorts the batch
+    // again from its (untouched) input, when the first levels were not enough
     if (rc == TURTLE_OK) rc = turtle_synchronize(s->h, nullptr);
-    if (rc == TURTLE_ERR_QUEUE_OVERFLOW) {
-        // the tables were grown by turtle_synchronize; the input is intact
-        rc = turtle_segmented_sort_device(s->h, s->dkin, any_idx ? s->diin : nullptr, s->dkout,
-                                          any_idx ? s->diout : nullptr, off.data(), (int64_t)batch.size(), s->stream);
-        if (rc == TURTLE_OK) rc = turtle_synchronize(s->h, nullptr);
-    }
     if (rc != TURTLE_OK) return rc;
     if (hipMemcpyAsync(s->pin_keys, s->dkout, (size_t)total * kb, hipMemcpyDeviceToHost, s->stream) != hipSuccess)
         return TURTLE_ERR_DEVICE;
