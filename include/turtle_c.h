@@ -148,6 +148,24 @@ int turtle_multisort_index(turtle_handle h, int dtype, int64_t arr_size, int64_t
 int turtle_sort_device(turtle_handle h, const void *keys_in, const int32_t *idx_in,
                        void *keys_out, int32_t *idx_out, int64_t n, void *stream);
 
+/* Allocation-free form of turtle_sort_device (no reference counterpart;
+ * for callers that cannot allocate device memory: tight serving loops and
+ * harnesses that forbid hipMalloc).  No handle: every buffer of the run
+ * beyond the caller's inputs and outputs lives in a static device array of
+ * the library (42.4 MiB, held on each GPU it runs on until the process
+ * ends), so at most 1,048,576 keys (TURTLE_ERR_CAPACITY above that).  Calls
+ * no hipMalloc, hipFree or other allocating routine.  dtype is a
+ * turtle_dtype; idx_in/idx_out are both NULL or both set; keys_out may
+ * equal keys_in (and idx_out idx_in).  Runs on `stream` (NULL = null
+ * stream) with splitters 31 and the largest LDS cutoff, and returns when the
+ * sort is done, with the same output as turtle_sort_device.  Calls on one
+ * GPU share its array and are serialised by a lock; the array keeps the
+ * last call's contents.  Not for use under stream capture (it reads back
+ * from the device).  A work-table overflow is reported as
+ * TURTLE_ERR_QUEUE_OVERFLOW (there is nothing to grow). */
+int turtle_sort_device_noalloc(int dtype, const void *keys_in, const int32_t *idx_in,
+                               void *keys_out, int32_t *idx_out, int64_t n, void *stream);
+
 /* Batched uneven segments in one engine run (no reference
  * counterpart; the reference's only route is a loop of StartSorting+Sync).
  * Segment s is [offsets[s], offsets[s+1]) of the concatenated arrays;

@@ -1413,6 +1413,177 @@ int turtle_graph_launch(turtle_graph_handle g, void *stream)
     return TURTLE_OK;
 }
 
+}  // extern "C"
+
+// ---- allocation-free sort -------------------------------------------------------
+// turtle_sort_device_noalloc keeps every buffer of an engine run beyond the
+// caller's inputs and outputs in `scratch`, a file-scope device array sized
+// for NA_MAX_N keys of any width with an index: the ping-pong key/index
+// buffers, a copy of an in-place call's input, the control block, segment,
+// tile and LDS-task tables, level records and per-key bins.  The tables are
+// sized as ensure_tables sizes them for NA_MAX_N keys in one root at the
+// smallest cutoff and tile (8-byte keys: 4096).  Host code reaches the
+// array only through the address hipGetSymbolAddress returns for the
+// current device (kept per device); the array's name is not a device
+// address on the host.
+namespace {
+
+constexpr uint64_t NA_MAX_N = 1ull << 20;
+constexpr uint64_t NA_S = 31;
+constexpr uint64_t NA_CUT8 = 4096, NA_CUT4 = 8192;    // LDS cutoff = tile: 8-byte keys, narrower keys
+constexpr uint64_t NA_SEGCAP = 8 * (NA_MAX_N / NA_CUT8 + 1) + 1024 + 1;
+constexpr uint64_t NA_TILECAP = NA_MAX_N / NA_CUT8 + 1 + NA_SEGCAP;
+constexpr uint64_t NA_LDSCAP = NA_SEGCAP * (NA_S + 1) + 1 + 1024;
+constexpr uint64_t na_bins_tiles(uint64_t tile) { return NA_MAX_N / tile + NA_MAX_N / (tile + 1) + 2; }
+constexpr uint64_t NA_BINS = na_bins_tiles(NA_CUT8) * NA_CUT8 > na_bins_tiles(NA_CUT4) * NA_CUT4
+                                 ? na_bins_tiles(NA_CUT8) * NA_CUT8
+                                 : na_bins_tiles(NA_CUT4) * NA_CUT4;
+constexpr uint64_t na_al(uint64_t x) { return (x + 255) & ~255ull; }
+// byte offsets inside the scratch array
+constexpr uint64_t NA_O_KT0 = 0;
+constexpr uint64_t NA_O_KT1 = NA_O_KT0 + na_al(8 * NA_MAX_N);
+constexpr uint64_t NA_O_KIN = NA_O_KT1 + na_al(8 * NA_MAX_N);
+constexpr uint64_t NA_O_IT0 = NA_O_KIN + na_al(8 * NA_MAX_N);
+constexpr uint64_t NA_O_IT1 = NA_O_IT0 + na_al(4 * NA_MAX_N);
+constexpr uint64_t NA_O_IIN = NA_O_IT1 + na_al(4 * NA_MAX_N);
+constexpr uint64_t NA_O_CTRL = NA_O_IIN + na_al(4 * NA_MAX_N);
+constexpr uint64_t NA_O_SEGS = NA_O_CTRL + na_al(sizeof(ts::Ctrl));
+constexpr uint64_t NA_O_TCNT = NA_O_SEGS + na_al(NA_SEGCAP * sizeof(ts::Seg));
+constexpr uint64_t NA_O_TPRE = NA_O_TCNT + na_al(NA_TILECAP * ts::MAX_BINS * 4);
+constexpr uint64_t NA_O_SPART = NA_O_TPRE + na_al(NA_TILECAP * ts::MAX_BINS * 4);
+constexpr uint64_t NA_O_TSEG = NA_O_SPART + na_al((NA_TILECAP * ts::MAX_BINS / ts::SCAN_CHUNK + 2) * 4);
+constexpr uint64_t NA_O_LDST = NA_O_TSEG + na_al(NA_TILECAP * 4);
+constexpr uint64_t NA_O_LDSQ = NA_O_LDST + na_al(NA_LDSCAP * sizeof(ts::LdsTask));
+constexpr uint64_t NA_O_LEVELS = NA_O_LDSQ + na_al(NA_LDSCAP * 4);
+constexpr uint64_t NA_O_BINS = NA_O_LEVELS + na_al(ts::MAX_LEVELS * sizeof(ts::Level));
+constexpr uint64_t NA_BYTES = NA_O_BINS + na_al(NA_BINS);
+static_assert(NA_BYTES < (64ull << 20), "the allocation-free scratch stays below 64 MiB");
+
+constexpr int NA_MAX_DEVICES = 64;
+struct NoAllocDevice {
+    std::mutex mu;                      // serialises the calls on this device
+    unsigned char *base = nullptr;      // device address of `scratch` on this device
+    int cus = 0;
+};
+NoAllocDevice g_na[NA_MAX_DEVICES];
+std::mutex g_na_init;
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+namespace {
+
+// address of this device's scratch array (looked up once per device, with
+// that device current)
+int na_device(int dev, NoAllocDevice **out)
+{
+    if (dev < 0 || dev >= NA_MAX_DEVICES) return fail(TURTLE_ERR_UNSUPPORTED, "device ordinal out of range");
+    std::lock_guard<std::mutex> lk(g_na_init);
+    NoAllocDevice &d = g_na[dev];
+    if (!d.base) {
+        void *ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        int cus = 0;
+        HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+        d.cus = cus;
+        d.base = (unsigned char *)ptr;
+    }
+    *out = &d;
+    return TURTLE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int turtle_sort_device_noalloc(int dtype, const void *keys_in, const int32_t *idx_in, void *keys_out,
+                               int32_t *idx_out, int64_t n, void *stream)
+{
+    const int kb = dtype_bytes(dtype);
+    if (!kb) return fail(TURTLE_ERR_DTYPE, "unknown dtype");
+    if (n < 0) return fail(TURTLE_ERR_INVALID, "negative n");
+    if ((uint64_t)n > NA_MAX_N) return fail(TURTLE_ERR_CAPACITY, "the allocation-free sort takes at most 1,048,576 keys");
+    if ((idx_in == nullptr) != (idx_out == nullptr)) return fail(TURTLE_ERR_INVALID, "idx_in and idx_out must both be set");
+    if (n == 0) return TURTLE_OK;
+    if (!keys_in || !keys_out) return fail(TURTLE_ERR_INVALID, "NULL key buffer");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev), "hipGetDevice");
+    NoAllocDevice *d = nullptr;
+    int rc = na_device(dev, &d);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(d->mu);
+    hipStream_t st = (hipStream_t)stream;
+    unsigned char *b = d->base;
+    const bool idx = idx_in != nullptr;
+    const void *kin = keys_in;
+    const uint32_t *iin = (const uint32_t *)idx_in;
+    // an in-place call reads its input from a copy (the levels write the output)
+    if (keys_in == keys_out) {
+        HIPCHK(hipMemcpyAsync(b + NA_O_KIN, keys_in, (size_t)n * kb, hipMemcpyDeviceToDevice, st), "hipMemcpyAsync");
+        kin = b + NA_O_KIN;
+    }
+    if (idx && (const void *)idx_in == (const void *)idx_out) {
+        HIPCHK(hipMemcpyAsync(b + NA_O_IIN, idx_in, (size_t)n * 4, hipMemcpyDeviceToDevice, st), "hipMemcpyAsync");
+        iin = (const uint32_t *)(b + NA_O_IIN);
+    }
+    const uint32_t cutoff = ts::engine_cap(kb), tile = ts::engine_tile(kb);
+    ts::Params p{};
+    p.kin = kin;
+    p.iin = idx ? iin : nullptr;
+    p.kout = keys_out;
+    p.iout = idx ? (uint32_t *)idx_out : nullptr;
+    p.kt0 = b + NA_O_KT0;
+    p.kt1 = b + NA_O_KT1;
+    p.it0 = (uint32_t *)(b + NA_O_IT0);
+    p.it1 = (uint32_t *)(b + NA_O_IT1);
+    p.ctrl = (ts::Ctrl *)(b + NA_O_CTRL);
+    p.segs = (ts::Seg *)(b + NA_O_SEGS);
+    p.tcnt = (uint32_t *)(b + NA_O_TCNT);
+    p.tpre = (uint32_t *)(b + NA_O_TPRE);
+    p.spart = (uint32_t *)(b + NA_O_SPART);
+    p.tile_seg = (uint32_t *)(b + NA_O_TSEG);
+    p.ldst = (ts::LdsTask *)(b + NA_O_LDST);
+    p.ldsq = (uint32_t *)(b + NA_O_LDSQ);
+    p.levels = (ts::Level *)(b + NA_O_LEVELS);
+    p.bins = b + NA_O_BINS;
+    p.segcap = (uint32_t)NA_SEGCAP;
+    p.tilecap = (uint32_t)NA_TILECAP;
+    p.ldscap = (uint32_t)NA_LDSCAP;
+    p.bins_tiles = (uint32_t)na_bins_tiles(tile);
+    p.cutoff = cutoff;
+    p.tiles_per_wg = 1;
+    p.n_total = (uint32_t)n;
+    p.flip = flip_of(dtype);
+    const int kt = kt_of(dtype);
+    // the levels in batches (the control block is cleared by the first
+    // batch's reset kernel), a small read-back after each, then the LDS
+    // sorts once
+    uint32_t nlev = levels_estimate((uint64_t)n, cutoff, NA_S), L0 = 0;
+    ts::Level lv{};
+    ts::Ctrl c{};
+    for (bool first = true;; first = false) {
+        if (L0 + nlev > (uint32_t)ts::MAX_LEVELS) nlev = ts::MAX_LEVELS - L0;
+        HIPCHK(ts::engine_levels(p, kt, idx, (int)NA_S, d->cus, 0, nullptr, 1, (uint32_t)n, L0, nlev, first, false, st),
+               "engine launch");
+        L0 += nlev;
+        if (L0 == 0) break;   // every key fits one LDS sort
+        HIPCHK(hipMemcpyAsync(&lv, p.levels + (L0 - 1), sizeof lv, hipMemcpyDeviceToHost, st), "D2H(level)");
+        HIPCHK(hipMemcpyAsync(&c, p.ctrl, sizeof c, hipMemcpyDeviceToHost, st), "D2H(ctrl)");
+        HIPCHK(hipStreamSynchronize(st), "hipStreamSynchronize");
+        const uint32_t left = c.seg_alloc > lv.se ? c.seg_alloc - lv.se : 0u;
+        if (left == 0 || c.error != 0 || L0 >= (uint32_t)ts::MAX_LEVELS) break;
+        nlev = 4;
+    }
+    HIPCHK(ts::engine_levels(p, kt, idx, (int)NA_S, d->cus, 0, nullptr, 1, (uint32_t)n, L0, 0, false, true, st),
+           "engine launch");
+    HIPCHK(hipMemcpyAsync(&c, p.ctrl, sizeof c, hipMemcpyDeviceToHost, st), "D2H(ctrl)");
+    HIPCHK(hipStreamSynchronize(st), "hipStreamSynchronize");
+    uint32_t err = c.error;
+    if (L0 > 0 && c.seg_alloc > lv.se) err |= ts::ERR_DEPTH;
+    return error_status(err);
+}
+
 int turtle_fill_keys(int dtype, void *keys, int64_t n, uint64_t seed, int shift, void *stream)
 {
     const int kb = dtype_bytes(dtype);
