@@ -166,6 +166,21 @@ int turtle_sort_device(turtle_handle h, const void *keys_in, const int32_t *idx_
 int turtle_sort_device_noalloc(int dtype, const void *keys_in, const int32_t *idx_in,
                                void *keys_out, int32_t *idx_out, int64_t n, void *stream);
 
+/* Device-wide stable merge of two sorted runs: replaces mergeSortedChunks
+ * (turtle-sort.cu:190-281), whose per-element rank rule puts a run-1 key
+ * before equal run-2 keys (upper bound of a run-2 key in run 1, lower bound
+ * of a run-1 key in run 2), and the chunk merges of Sync
+ * (turtle-sort.cuh:192-243).  keys_a[0..na) and keys_b[0..nb) are sorted
+ * ascending (in the key type's order, floating-point keys in totalOrder);
+ * keys_out[0..na+nb) receives their stable merge.  idx_a, idx_b and idx_out
+ * are all NULL or all set: a 32-bit value moves with every key (the
+ * reference's merge writes keys into its index array, turtle-sort.cu:220,
+ * 255; here the index is carried).  The output must not overlap the runs.
+ * Asynchronous on `stream` (NULL = null stream). */
+int turtle_merge_device(int dtype, const void *keys_a, const int32_t *idx_a, int64_t na,
+                        const void *keys_b, const int32_t *idx_b, int64_t nb,
+                        void *keys_out, int32_t *idx_out, void *stream);
+
 /* Batched uneven segments in one engine run (no reference
  * counterpart; the reference's only route is a loop of StartSorting+Sync).
  * Segment s is [offsets[s], offsets[s+1]) of the concatenated arrays;

@@ -119,3 +119,21 @@ def cpu_sort(which, keys, idx=None, threads=0, want_output=False):
     sec = L.ts_cpu_sort(which, DTYPES[k.dtype], _p(k), _p(idx), k.size, threads, _p(ok), _p(oi))
     assert sec >= 0
     return (sec, ok, oi) if want_output else sec
+
+
+def merge_ranks(k1, i1, k2, i2):
+    """The reference's stable rank merge of two sorted runs
+    (turtle-sort.cu:190-281): run-1 keys go before equal run-2 keys.  Keys of
+    any dtype (compared by their order values); i1/i2 may be None."""
+    L = load()
+    a, b = np.ascontiguousarray(k1), np.ascontiguousarray(k2)
+    oa, ob = order(a), order(b)
+    ko = np.empty(a.size + b.size, dtype=np.uint64)
+    with_idx = i1 is not None
+    ia = np.ascontiguousarray(i1, dtype=np.int32) if with_idx else None
+    ib = np.ascontiguousarray(i2, dtype=np.int32) if with_idx else None
+    io = np.empty(a.size + b.size, dtype=np.int32) if with_idx else None
+    L.ts_oracle_merge_ranks(_p(oa), _p(ia), a.size, _p(ob), _p(ib), b.size, _p(ko), _p(io))
+    out = np.empty(a.size + b.size, dtype=a.dtype)
+    assert L.ts_oracle_from_ord(DTYPES[a.dtype], _p(ko), _p(out), out.size) == 0
+    return out, io

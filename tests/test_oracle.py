@@ -397,3 +397,29 @@ def test_float_order_map_brute_small(orc, dtype):
     for i in range(x.size):
         for j in range(x.size):
             assert (m[i] < m[j]) == (x[i] < x[j])
+
+
+def test_merge_ranks_typed_wrapper(orc):
+    """oracle.merge_ranks over every key type: the stable merge equals a
+    stable sort of run 1 followed by run 2 (by the key type's order), and
+    the payload follows (run 1 first on equal keys)."""
+    rng = np.random.default_rng(12)
+    for dt in [np.int8, np.uint8, np.int16, np.uint16, np.int32, np.uint32, np.int64, np.uint64, np.float32,
+               np.float64]:
+        for n1, n2 in [(0, 0), (0, 5), (7, 0), (33, 50), (300, 17)]:
+            if np.dtype(dt).kind == "f":
+                raw = rng.integers(-4, 4, n1 + n2).astype(dt) / 2
+            else:
+                info = np.iinfo(dt)
+                raw = rng.integers(max(info.min, -6), min(info.max, 6), n1 + n2).astype(dt)
+            a, b = np.sort(raw[:n1]), np.sort(raw[n1:])
+            ia = np.arange(n1, dtype=np.int32)
+            ib = np.arange(n1, n1 + n2, dtype=np.int32)
+            ko, io = orc.merge_ranks(a, ia, b, ib)
+            cat = np.concatenate([a, b])
+            perm = np.argsort(cat, kind="stable")
+            np.testing.assert_array_equal(ko, cat[perm])
+            np.testing.assert_array_equal(io, perm.astype(np.int32))
+            k2, i2 = orc.merge_ranks(a, None, b, None)
+            assert i2 is None
+            np.testing.assert_array_equal(k2, cat[perm])

@@ -117,6 +117,7 @@ SIGNATURES = {
     "turtle_multisort": (_i32, [_vp, _i32, _i64, _i32, _i32, _i64, _vp, ctypes.POINTER(ctypes.c_double)]),
     "turtle_sort_device": (_i32, [_vp, _vp, _vp, _vp, _vp, _i64, _vp]),
     "turtle_sort_device_noalloc": (_i32, [_i32, _vp, _vp, _vp, _vp, _i64, _vp]),
+    "turtle_merge_device": (_i32, [_i32, _vp, _vp, _i64, _vp, _vp, _i64, _vp, _vp, _vp]),
     "turtle_segmented_sort_device": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp]),
     "turtle_multisort_device": (_i32, [_vp, _i32, _i64, _vp, _vp, _i64, _vp]),
     "turtle_multisort_index": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _i32, ctypes.POINTER(ctypes.c_double)]),

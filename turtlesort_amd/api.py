@@ -339,6 +339,27 @@ class SegmentGraph:
             pass
 
 
+def merge_device(keys_a, keys_b, keys_out, idx_a=None, idx_b=None, idx_out=None, dtype=None, stream=None):
+    """Stable merge of two sorted device runs into keys_out (turtle_merge_device):
+    the reference's mergeSortedChunks rank rule, run a first on equal keys.
+    idx_a, idx_b, idx_out: all None or all int32 tensors moved with the keys.
+    dtype: the key type (default: the tensors' dtype; give it when the keys
+    are a same-width bit view, e.g. uint32 keys in an int32 tensor)."""
+    lib = _lib.load()
+    code = dtype_code(keys_a.dtype if dtype is None else dtype)
+    na, nb = keys_a.numel(), keys_b.numel()
+    if (idx_a is None) != (idx_out is None) or (idx_b is None) != (idx_out is None):
+        raise ValueError("idx_a, idx_b and idx_out must all be given or all be None")
+    specs = [(keys_a, "keys_a", code, na, False), (keys_b, "keys_b", code, nb, False),
+             (keys_out, "keys_out", code, na + nb, False)]
+    if idx_out is not None:
+        specs += [(idx_a, "idx_a", None, na, True), (idx_b, "idx_b", None, nb, True),
+                  (idx_out, "idx_out", None, na + nb, True)]
+    _check_tensors(specs)
+    _lib.check(lib.turtle_merge_device(code, _ptr(keys_a), _ptr(idx_a), na, _ptr(keys_b), _ptr(idx_b), nb,
+                                       _ptr(keys_out), _ptr(idx_out), _stream(stream)))
+
+
 def fill_keys(keys, seed, shift, stream=None):
     """keys[i] = splitmix64 draw i of seed >> shift (device tensor, in place)."""
     lib = _lib.load()

@@ -1584,6 +1584,27 @@ int turtle_sort_device_noalloc(int dtype, const void *keys_in, const int32_t *id
     return error_status(err);
 }
 
+int turtle_merge_device(int dtype, const void *keys_a, const int32_t *idx_a, int64_t na, const void *keys_b,
+                        const int32_t *idx_b, int64_t nb, void *keys_out, int32_t *idx_out, void *stream)
+{
+    const int kb = dtype_bytes(dtype);
+    if (!kb) return fail(TURTLE_ERR_DTYPE, "unknown dtype");
+    if (na < 0 || nb < 0) return fail(TURTLE_ERR_INVALID, "negative length");
+    if (na + nb >= ((int64_t)1 << 40)) return fail(TURTLE_ERR_CAPACITY, "runs too long");
+    if (na + nb == 0) return TURTLE_OK;
+    // an empty run's pointers may be NULL
+    const bool idx = idx_out != nullptr;
+    if (idx ? ((na && !idx_a) || (nb && !idx_b)) : (idx_a || idx_b))
+        return fail(TURTLE_ERR_INVALID, "idx_a, idx_b and idx_out must all be set or all be NULL");
+    if ((na && !keys_a) || (nb && !keys_b) || !keys_out) return fail(TURTLE_ERR_INVALID, "NULL key buffer");
+    if (keys_out == keys_a || keys_out == keys_b || (idx && (idx_out == idx_a || idx_out == idx_b)))
+        return fail(TURTLE_ERR_INVALID, "the output must not overlap the runs");
+    HIPCHK(ts::merge_run(kt_of(dtype), flip_of(dtype), keys_a, (const uint32_t *)idx_a, (uint64_t)na, keys_b,
+                         (const uint32_t *)idx_b, (uint64_t)nb, keys_out, (uint32_t *)idx_out, (hipStream_t)stream),
+           "merge launch");
+    return TURTLE_OK;
+}
+
 int turtle_fill_keys(int dtype, void *keys, int64_t n, uint64_t seed, int shift, void *stream)
 {
     const int kb = dtype_bytes(dtype);

@@ -39,6 +39,12 @@ hipError_t multisort_run(int kt, uint64_t flip, uint32_t arr_size, const void *i
                          uint32_t *iout, uint64_t num_arrays, hipStream_t st);
 uint32_t multisort_max_size(int kb);
 
+// Stable merge of two sorted runs (ts_merge.hip): out[0, na+nb) = a merged
+// with b, a's elements first among equal keys; ia/ib/iout (all or none) move
+// a 32-bit payload with every key.
+hipError_t merge_run(int kt, uint64_t flip, const void *a, const uint32_t *ia, uint64_t na, const void *b,
+                     const uint32_t *ib, uint64_t nb, void *out, uint32_t *iout, hipStream_t st);
+
 // synthetic inputs (ts_util.hip)
 hipError_t fill_keys(int kb, void *keys, uint64_t n, uint64_t seed, int shift, hipStream_t st);
 hipError_t fill_iota(uint32_t *idx, uint64_t n, hipStream_t st, uint64_t period = 0);   // period > 0: i mod period

@@ -9,52 +9,198 @@
 // a multiple of the reference's BlockSize (turtle-multi-sort.cuh:200-202
 // skips it), and the input buffer is the one that is sorted
 // (turtle-multi-sort.cuh:152 sorts an unfilled scratch buffer).
+#include <type_traits>
+
 #include "ts_device.h"
 #include "ts_internal.h"
 
 namespace ts {
 
 // IDX: a 32-bit payload per key moves with it (stably); iin == nullptr
-// gives each key its position within its array.
+// gives each key its position within its array.  A workgroup sorts
+// `per_wg` consecutive arrays, each in its own power-of-two region of
+// n_eff = next_pow2(arr_size) slots (per_wg * n_eff <= NT * ITEMS): short
+// arrays are packed so that every lane has keys to sort.
 template <typename U, bool FL, bool IDX, int NT, int ITEMS>
 __global__ void __launch_bounds__(NT) ms_kernel(const U *__restrict__ in, U *__restrict__ out,
                                                  const uint32_t *__restrict__ iin, uint32_t *__restrict__ iout,
-                                                 uint32_t arr_size, U flip)
+                                                 uint32_t arr_size, uint32_t per_wg, uint64_t num_arrays, U flip)
 {
     constexpr uint32_t CAPA = (uint32_t)NT * ITEMS;
     __shared__ U sk[padded_len<ITEMS>(CAPA)];
     __shared__ uint32_t si[IDX ? padded_len<ITEMS>(CAPA) : 1];
-    const size_t base = (size_t)blockIdx.x * arr_size;
+    const uint64_t arr0 = (uint64_t)blockIdx.x * per_wg;
+    const uint32_t nar = num_arrays - arr0 < per_wg ? (uint32_t)(num_arrays - arr0) : per_wg;
     uint32_t n_eff = next_pow2_dev(arr_size);
     if (n_eff < (uint32_t)ITEMS) n_eff = ITEMS;
-    // n_eff <= NT*ITEMS: ITEMS strided positions per thread; all loads are
-    // issued before any is used, so that their latencies overlap
+    const uint32_t lg = 31 - __clz(n_eff);
+    const uint32_t tot = per_wg * n_eff;   // slots in use: per_wg regions
+    const size_t base = (size_t)arr0 * arr_size;
+    // ITEMS strided slots per thread; slot s is key s & (n_eff-1) of the
+    // workgroup's array s >> lg.  All loads are issued before any is used,
+    // so that their latencies overlap
     U kv[ITEMS];
     uint32_t iv[ITEMS];
 #pragma unroll
     for (int r = 0; r < ITEMS; r++) {
-        const uint32_t i = threadIdx.x + (uint32_t)r * NT;
-        kv[r] = i < arr_size ? in[base + i] : (U)0;
-        iv[r] = IDX && iin && i < arr_size ? iin[base + i] : i;
+        const uint32_t s = threadIdx.x + (uint32_t)r * NT;
+        const uint32_t q = s >> lg, i = s & (n_eff - 1);
+        const bool ok = q < nar && i < arr_size;
+        const size_t g = base + (size_t)q * arr_size + i;
+        kv[r] = ok ? in[g] : (U)0;
+        iv[r] = IDX && iin && ok ? iin[g] : i;
     }
 #pragma unroll
     for (int r = 0; r < ITEMS; r++) {
-        const uint32_t i = threadIdx.x + (uint32_t)r * NT;
-        if (i < n_eff) {
-            sk[pidx<ITEMS>(i)] = i < arr_size ? to_ord<U, FL>(kv[r], flip) : key_max<U>();
-            if (IDX) si[pidx<ITEMS>(i)] = iv[r];
+        const uint32_t s = threadIdx.x + (uint32_t)r * NT;
+        const uint32_t q = s >> lg, i = s & (n_eff - 1);
+        if (s < tot) {
+            sk[pidx<ITEMS>(s)] = q < nar && i < arr_size ? to_ord<U, FL>(kv[r], flip) : key_max<U>();
+            if (IDX) si[pidx<ITEMS>(s)] = iv[r];
         }
     }
     block_sync();
-    block_sort<U, IDX, NT, ITEMS>(sk, si, n_eff, n_eff);
+    block_sort<U, IDX, NT, ITEMS>(sk, si, tot, n_eff);
 #pragma unroll
     for (int r = 0; r < ITEMS; r++) {
-        const uint32_t i = threadIdx.x + (uint32_t)r * NT;
-        if (i < arr_size) {
-            out[base + i] = from_ord<U, FL>(sk[pidx<ITEMS>(i)], flip);
-            if (IDX) iout[base + i] = si[pidx<ITEMS>(i)];
+        const uint32_t s = threadIdx.x + (uint32_t)r * NT;
+        const uint32_t q = s >> lg, i = s & (n_eff - 1);
+        if (q < nar && i < arr_size) {
+            const size_t g = base + (size_t)q * arr_size + i;
+            out[g] = from_ord<U, FL>(sk[pidx<ITEMS>(s)], flip);
+            if (IDX) iout[g] = si[pidx<ITEMS>(s)];
         }
     }
+}
+
+// ---- arrays of at most 32 keys: one lane per array --------------------------------
+// The reference's MultiSort gives each thread one array (its heap sort,
+// turtle-multi-sort.cuh:86-182); for arrays this short that is the right
+// grain.  Here a workgroup of NT lanes takes NT consecutive arrays: their
+// keys are loaded with coalesced loads into LDS (array q at q*P, P = the
+// array length rounded up to an odd number of words, so that lanes reading
+// their own arrays hit distinct banks), each lane sorts its array in
+// registers with Batcher's odd-even merge network of N = next_pow2(len)
+// wires (keys past the length are the largest key), and the arrays go back
+// through LDS with coalesced stores.  With an index the network compares
+// (key, position) pairs, a total order, so the result is the stable order,
+// and the payload is gathered by the sorted positions.
+// Order values are held as W (32 bits for keys of up to 4 bytes).
+template <typename W, int N, bool IDX>
+__device__ __forceinline__ void cmpx(W (&k)[N], uint32_t (&p)[N], int i, int j)
+{
+    if constexpr (IDX) {
+        const bool sw = k[j] < k[i] || (k[j] == k[i] && p[j] < p[i]);
+        const W a = k[i], b = k[j];
+        const uint32_t x = p[i], y = p[j];
+        k[i] = sw ? b : a;
+        k[j] = sw ? a : b;
+        p[i] = sw ? y : x;
+        p[j] = sw ? x : y;
+    } else {
+        const W a = k[i], b = k[j];
+        k[i] = a < b ? a : b;
+        k[j] = a < b ? b : a;
+    }
+}
+
+// Batcher's odd-even merge sort on N (a power of two) registers
+template <typename W, int N, bool IDX>
+__device__ __forceinline__ void oddeven_merge_sort(W (&k)[N], uint32_t (&p)[N])
+{
+#pragma unroll
+    for (int q = 1; q < N; q <<= 1)
+#pragma unroll
+        for (int d = q; d >= 1; d >>= 1)
+#pragma unroll
+            for (int j = d % q; j + d < N; j += 2 * d)
+#pragma unroll
+                for (int i = 0; i < d; i++)
+                    if (i + j + d < N && (i + j) / (2 * q) == (i + j + d) / (2 * q)) cmpx<W, N, IDX>(k, p, i + j, i + j + d);
+}
+
+template <typename U, typename W, bool FL, bool IDX, int N, int NT>
+__global__ void __launch_bounds__(NT) ms_small(const U *__restrict__ in, U *__restrict__ out,
+                                                const uint32_t *__restrict__ iin, uint32_t *__restrict__ iout,
+                                                uint32_t arr_size, uint64_t inv, uint64_t num_arrays, U flip)
+{
+    constexpr uint32_t PMAX = 33;
+    __shared__ W sk[NT * PMAX];
+    __shared__ uint32_t si[IDX ? NT * PMAX : 1];
+    const uint32_t A = arr_size, P = A | 1u;
+    const uint64_t arr0 = (uint64_t)blockIdx.x * NT;
+    const uint32_t nar = num_arrays - arr0 < (uint64_t)NT ? (uint32_t)(num_arrays - arr0) : (uint32_t)NT;
+    const uint32_t ne = nar * A;
+    const size_t base = (size_t)arr0 * A;
+    const uint32_t t = threadIdx.x;
+    // coalesced load, element e of the workgroup's run = key e % A of its
+    // array e / A (e / A = (e * inv) >> 32 with inv = ceil(2^32 / A), exact
+    // for e < 2^32 / A); eight loads in flight per step
+    for (uint32_t r0 = 0; r0 < A; r0 += 8) {
+        U kv[8];
+        uint32_t iv[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const uint32_t e = t + (r0 + u) * NT;
+            const bool ok = r0 + u < A && e < ne;
+            kv[u] = ok ? in[base + e] : (U)0;
+            iv[u] = IDX && iin && ok ? iin[base + e] : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const uint32_t e = t + (r0 + u) * NT;
+            if (r0 + u < A && e < ne) {
+                const uint32_t q = (uint32_t)(((uint64_t)e * inv) >> 32), j = e - q * A;
+                sk[q * P + j] = (W)to_ord<U, FL>(kv[u], flip);
+                if (IDX) si[q * P + j] = iin ? iv[u] : j;
+            }
+        }
+    }
+    block_sync();
+    W k[N];
+    uint32_t p[N];
+    const bool act = t < nar;
+#pragma unroll
+    for (int j = 0; j < N; j++) {
+        k[j] = act && (uint32_t)j < A ? sk[t * P + j] : (W)key_max<U>();
+        p[j] = (uint32_t)j;
+    }
+    oddeven_merge_sort<W, N, IDX>(k, p);
+    uint32_t v[N];
+    if (IDX) {
+#pragma unroll
+        for (int j = 0; j < N; j++) v[j] = act && (uint32_t)j < A ? si[t * P + (p[j] < A ? p[j] : 0u)] : 0u;
+    }
+    block_sync();
+    if (act) {
+#pragma unroll
+        for (int j = 0; j < N; j++) {
+            if ((uint32_t)j < A) {
+                sk[t * P + j] = k[j];
+                if (IDX) si[t * P + j] = v[j];
+            }
+        }
+    }
+    block_sync();
+    for (uint32_t r = 0; r < A; r++) {
+        const uint32_t e = t + r * NT;
+        if (e < ne) {
+            const uint32_t q = (uint32_t)(((uint64_t)e * inv) >> 32), j = e - q * A;
+            out[base + e] = from_ord<U, FL>((U)sk[q * P + j], flip);
+            if (IDX) iout[base + e] = si[q * P + j];
+        }
+    }
+}
+
+template <typename U, bool FL, bool IDX, int N>
+static void small_launch(uint32_t arr_size, const U *in, U *out, const uint32_t *iin, uint32_t *iout, uint64_t na,
+                         U f, hipStream_t st)
+{
+    typedef typename std::conditional<sizeof(U) == 8, uint64_t, uint32_t>::type W;
+    constexpr int NT = sizeof(U) == 8 ? 128 : 256;
+    const uint64_t inv = ((1ull << 32) + arr_size - 1) / arr_size;
+    hipLaunchKernelGGL((ms_small<U, W, FL, IDX, N, NT>), dim3((uint32_t)((na + NT - 1) / NT)), dim3(NT), 0, st, in, out,
+                       iin, iout, arr_size, inv, na, f);
 }
 
 template <typename U, bool FL, bool IDX>
@@ -74,28 +220,31 @@ static hipError_t ms_launch(uint64_t flip, uint32_t arr_size, const void *in, vo
         U *oo = o + e0;
         const uint32_t *xi = iin ? iin + e0 : nullptr;
         uint32_t *xo = IDX ? iout + e0 : nullptr;
-        // one workgroup per array, sized so that every thread holds ITEMS
-        // keys of it (one wave up to 1024 keys: no cross-wave barriers, and
-        // 16 keys per thread take one merge round fewer than 8; 65,536 x
-        // 1024 int32 keys: 0.34 -> 0.29 ms)
-        if (ne <= 256)
-            hipLaunchKernelGGL((ms_kernel<U, FL, IDX, 64, 4>), dim3((uint32_t)na), dim3(64), 0, st, ii, oo, xi, xo,
-                               arr_size, f);
-        else if (ne <= 512)
-            hipLaunchKernelGGL((ms_kernel<U, FL, IDX, 64, 8>), dim3((uint32_t)na), dim3(64), 0, st, ii, oo, xi, xo,
-                               arr_size, f);
-        else if (ne <= 1024)
-            hipLaunchKernelGGL((ms_kernel<U, FL, IDX, 64, 16>), dim3((uint32_t)na), dim3(64), 0, st, ii, oo, xi, xo,
-                               arr_size, f);
-        else if (ne <= 2048)
+        if (ne <= 32) {
+            // one lane per array, a register network of ne wires
+            switch (ne < 2 ? 2 : ne) {
+            case 2: small_launch<U, FL, IDX, 2>(arr_size, ii, oo, xi, xo, na, f, st); break;
+            case 4: small_launch<U, FL, IDX, 4>(arr_size, ii, oo, xi, xo, na, f, st); break;
+            case 8: small_launch<U, FL, IDX, 8>(arr_size, ii, oo, xi, xo, na, f, st); break;
+            case 16: small_launch<U, FL, IDX, 16>(arr_size, ii, oo, xi, xo, na, f, st); break;
+            default: small_launch<U, FL, IDX, 32>(arr_size, ii, oo, xi, xo, na, f, st); break;
+            }
+        } else if (ne <= 1024) {
+            // one wave of 16 keys per lane, 1024 / ne arrays per wave (no
+            // cross-wave barriers; 16 keys per thread take one merge round
+            // fewer than 8; 65,536 x 1024 int32 keys: 0.34 -> 0.29 ms)
+            const uint32_t per = 1024u / ne;
+            hipLaunchKernelGGL((ms_kernel<U, FL, IDX, 64, 16>), dim3((uint32_t)((na + per - 1) / per)), dim3(64), 0, st,
+                               ii, oo, xi, xo, arr_size, per, na, f);
+        } else if (ne <= 2048)
             hipLaunchKernelGGL((ms_kernel<U, FL, IDX, 128, 16>), dim3((uint32_t)na), dim3(128), 0, st, ii, oo, xi,
-                               xo, arr_size, f);
+                               xo, arr_size, 1u, na, f);
         else if (ne <= 4096)
             hipLaunchKernelGGL((ms_kernel<U, FL, IDX, 256, 16>), dim3((uint32_t)na), dim3(256), 0, st, ii, oo, xi,
-                               xo, arr_size, f);
+                               xo, arr_size, 1u, na, f);
         else if (ne <= 8192)
             hipLaunchKernelGGL((ms_kernel<U, FL, IDX, 512, 16>), dim3((uint32_t)na), dim3(512), 0, st, ii, oo, xi,
-                               xo, arr_size, f);
+                               xo, arr_size, 1u, na, f);
         else
             return hipErrorInvalidValue;
         hipError_t e = hipGetLastError();
