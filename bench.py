@@ -278,7 +278,88 @@ def bench_small_configs(torch, np, wl, TurtleSort, fill_keys, fill_iota, reps=20
         "pcie_bytes": 2 * 4 * num * arr, "seconds": [round(t, 4) for t in host_s],
         "valid": bool(np.array_equal(x.reshape(num, arr), d2h_np(torch, ref).reshape(num, arr)))}
     s.close()
+    # the reference's published MultiSort shapes (README.md:265-384): device
+    # kernel time, and the host-array call (transfers included) beside the
+    # README's 0.0253 s for 6,400,000 x 10 int on its GPU
+    for name, dt, tdt, arr, num, ref in (("multisort_6400000x10_int32", np.int32, torch.int32, 10, 6_400_000,
+                                          "README.md:327-345: 0.0253-0.0263 s wall incl. copies, NVIDIA GPU"),
+                                         ("multisort_3200000x32_uint8", np.uint8, torch.uint8, 32, 3_200_000,
+                                          "README.md:380-384: 2.7 ms kernel, NVIDIA GPU")):
+        n = arr * num
+        s = TurtleSort(dt, n, track_index=False)
+        kin = torch.empty(n, dtype=tdt, device="cuda")
+        fill_keys(kin, 4321, 0)
+        kout = torch.empty_like(kin)
+        s.multi_sort_device(arr, kin, kout)
+        s.synchronize()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record()
+        for _ in range(reps):
+            s.multi_sort_device(arr, kin, kout)
+        ev1.record()
+        torch.cuda.synchronize()
+        t = ev0.elapsed_time(ev1) / 1e3 / reps
+        ref_rows, _ = torch.sort(kin.view(num, arr), dim=1)
+        row = {"ms_per_launch": round(t * 1e3, 4), "Gkeys_s": round(n / t / 1e9, 3),
+               "hbm_frac": round(2 * kin.element_size() * n / t / 1e9 / HBM_PEAK_GBS, 4),
+               "valid": bool(torch.equal(kout.view(num, arr), ref_rows)), "reference": ref}
+        hd = d2h_np(torch, kin)
+        host_s = []
+        for _ in range(3):
+            x = hd.copy()
+            host_s.append(s.multi_sort(arr, x))
+        row["host_call_s"] = round(min(host_s), 4)
+        row["host_valid"] = bool(np.array_equal(x.reshape(num, arr), d2h_np(torch, ref_rows).reshape(num, arr)))
+        out[name] = row
+        s.close()
+        del kin, kout, ref_rows
+    # stable merge of two sorted 32M-key int32 + index runs (turtle_merge_device)
+    from turtlesort_amd import merge_device
+    m = 1 << 25
+    x = torch.empty(2 * m, dtype=torch.int32, device="cuda")
+    fill_keys(x, 99, wl.SHIFT_31BIT)
+    a, ia = torch.sort(x[:m])
+    b, ib = torch.sort(x[m:])
+    ia, ib = ia.to(torch.int32), (ib + m).to(torch.int32)
+    ko, io = torch.empty_like(x), torch.empty_like(x)
+    merge_device(a, b, ko, ia, ib, io)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(reps):
+        merge_device(a, b, ko, ia, ib, io)
+    ev1.record()
+    torch.cuda.synchronize()
+    t = ev0.elapsed_time(ev1) / 1e3 / reps
+    ok = bool((ko[1:] >= ko[:-1]).all()) and bool(torch.equal(x[io.long()], ko))
+    out["merge_2x32M_int32_idx"] = {"ms_per_launch": round(t * 1e3, 4), "Gkeys_s": round(2 * m / t / 1e9, 3),
+                                    "alg_bytes": 2 * 8 * 2 * m,
+                                    "hbm_frac": round(2 * 8 * 2 * m / t / 1e9 / HBM_PEAK_GBS, 4), "valid": ok}
+    del x, a, b, ia, ib, ko, io
     return out
+
+
+def bench_readme_startsorting(np, wl, TurtleSort):
+    """the reference's only published StartSorting benchmark shape
+    (README.md:86-251): 12,582,912 int keys + int index on host arrays,
+    StartSorting + Sync wall seconds (transfers included); its keys are
+    MSVC rand() values in [0, 32767]"""
+    n = 12_582_912
+    keys = wl.gen_keys(np.int32, n, wl.SEED_C1, wl.SHIFT_15BIT)
+    s = TurtleSort(np.int32, n)
+    secs = []
+    for _ in range(5):
+        k = keys.copy()
+        i = np.arange(n, dtype=np.int32)
+        s.start_sorting(k, i)
+        secs.append(s.sync())
+    s.close()
+    ok = bool((k[1:] >= k[:-1]).all()) and bool(np.array_equal(keys[i], k))
+    return {"workload": "StartSorting(&keys,&idx)+Sync(), 12,582,912 int32 keys in [0,32767] + int32 index, host "
+                        "arrays (PCIe included)", "seconds_min": round(min(secs), 4),
+            "seconds": [round(x, 4) for x in secs], "Gkeys_s": round(n / min(secs) / 1e9, 4), "valid": ok,
+            "reference": "README.md:224-251: 0.0323-0.0368 s (median 0.0331 s) on an NVIDIA RTX 4070/4090 "
+                         "with its host"}
 
 
 def d2h_np(torch, t):
@@ -444,6 +525,7 @@ def main():
 
     if not args.no_variants:
         out["small_configs"] = bench_small_configs(torch, np, wl, TurtleSort, fill_keys, fill_iota)
+        out["small_configs"]["readme_startsorting_12582912"] = bench_readme_startsorting(np, wl, TurtleSort)
 
     if not args.no_batched:
         del kin, kout, iin, iout
@@ -480,6 +562,27 @@ def main():
             "gnu_parallel_threads": cores,
             "host_hw_threads": lib.ts_cpu_hw_threads(),
         }
+        # the batch configurations: per-array / per-segment std::sort loops on
+        # bounded samples, one thread and omp parallel for on every core
+        na3 = 8192
+        k3 = wl.gen_keys(np.int32, na3 * wl.C3_SIZE, wl.SEED_C3, wl.SHIFT_31BIT)
+        off3 = np.arange(na3 + 1, dtype=np.int64) * wl.C3_SIZE
+        t3 = oracle.cpu_segmented_sort(0, k3, None, off3)
+        t3p = oracle.cpu_segmented_sort(1, k3, None, off3)
+        lens = wl.loguniform_lengths(wl.SEED_C4_LENGTHS, wl.C4_SEGMENTS, wl.C4_LO, wl.C4_HI)[:1000]
+        off4 = wl.segment_offsets(lens)
+        k4 = wl.gen_keys(np.int64, int(off4[-1]), wl.SEED_C4_KEYS, wl.SHIFT_63BIT)
+        i4 = np.arange(k4.size, dtype=np.int32)
+        t4 = oracle.cpu_segmented_sort(0, k4, i4, off4)
+        t4p = oracle.cpu_segmented_sort(1, k4, i4, off4)
+        out["cpu_baseline"]["c3_multisort"] = {
+            "sample": f"{na3} arrays x {wl.C3_SIZE} int32 uniform [0,2^31), same generator",
+            "std_sort_loop_1t_Gkeys_s": round(k3.size / t3 / 1e9, 5),
+            "std_sort_loop_omp_Gkeys_s": round(k3.size / t3p / 1e9, 5), "omp_threads": cores}
+        out["cpu_baseline"]["c4_batched"] = {
+            "sample": f"the first 1000 of the 20,000 segments ({k4.size} keys), int64 keys + int32 index",
+            "std_sort_loop_1t_Gkeys_s": round(k4.size / t4 / 1e9, 5),
+            "std_sort_loop_omp_Gkeys_s": round(k4.size / t4p / 1e9, 5), "omp_threads": cores}
     print(json.dumps(out))
     sorter.close()
 

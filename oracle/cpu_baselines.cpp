@@ -4,7 +4,9 @@
 // bench.py).  Each baseline sorts a {key, idx} struct by key, the way the
 // reference's benchmark program does (README.md:123-127, 176-181):
 //   std::sort (1 thread), std::sort(std::execution::par_unseq, ...) and the
-//   libstdc++ parallel-mode __gnu_parallel::sort on all cores (OpenMP).
+//   libstdc++ parallel-mode __gnu_parallel::sort on all cores (OpenMP);
+//   for batches of arrays/segments, a per-segment std::sort loop, plain and
+//   under omp parallel for.
 // libstdc++ selects a TBB backend for par_unseq only when <tbb/tbb.h> exists;
 // without it the PSTL serial backend runs, and ts_cpu_pstl_backend() says so.
 #include <algorithm>
@@ -47,9 +49,52 @@ double time_sort(int which, const K* keys, const int32_t* idx, int64_t n, int th
     return std::chrono::duration<double>(t1 - t0).count();
 }
 
+// Per-segment sorts of a batch (the CPU form of MultiSort and of the
+// batched uneven segments): segment s is [off[s], off[s+1]).  With idx, a
+// {key, idx} struct per element, else the keys alone.  par: the segments
+// are spread over `threads` OpenMP threads (dynamic schedule).
+template <typename K>
+double time_segmented(bool par, const K* keys, const int32_t* idx, const int64_t* off, int64_t nseg, int threads)
+{
+    const int64_t n = off[nseg];
+    std::vector<Rec<K>> v(idx ? (size_t)n : 0);
+    std::vector<K> k(idx ? 0 : (size_t)n);
+    if (idx) for (int64_t i = 0; i < n; i++) { v[i].key = keys[i]; v[i].idx = idx[i]; }
+    else std::memcpy(k.data(), keys, (size_t)n * sizeof(K));
+    auto cmp = [](const Rec<K>& a, const Rec<K>& b) { return a.key < b.key; };
+    omp_set_num_threads(par ? (threads > 0 ? threads : omp_get_num_procs()) : 1);
+    auto t0 = std::chrono::steady_clock::now();
+#pragma omp parallel for schedule(dynamic, 16) if (par)
+    for (int64_t s = 0; s < nseg; s++) {
+        if (idx) std::sort(v.begin() + off[s], v.begin() + off[s + 1], cmp);
+        else std::sort(k.begin() + off[s], k.begin() + off[s + 1]);
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    return std::chrono::duration<double>(t1 - t0).count();
+}
+
 }  // namespace
 
 extern "C" {
+
+// par: 0 a loop of std::sort over the segments on one thread, 1 the same
+// loop under omp parallel for.  Returns seconds spent sorting, < 0 on error.
+double ts_cpu_segmented_sort(int par, int dtype, const void* keys, const int32_t* idx, const int64_t* off,
+                             int64_t nseg, int threads)
+{
+    const bool p = par != 0;
+    switch (dtype) {
+    case 1: return time_segmented<int8_t>(p, (const int8_t*)keys, idx, off, nseg, threads);
+    case 2: return time_segmented<uint8_t>(p, (const uint8_t*)keys, idx, off, nseg, threads);
+    case 3: return time_segmented<int16_t>(p, (const int16_t*)keys, idx, off, nseg, threads);
+    case 4: return time_segmented<uint16_t>(p, (const uint16_t*)keys, idx, off, nseg, threads);
+    case 5: return time_segmented<int32_t>(p, (const int32_t*)keys, idx, off, nseg, threads);
+    case 6: return time_segmented<uint32_t>(p, (const uint32_t*)keys, idx, off, nseg, threads);
+    case 7: return time_segmented<int64_t>(p, (const int64_t*)keys, idx, off, nseg, threads);
+    case 8: return time_segmented<uint64_t>(p, (const uint64_t*)keys, idx, off, nseg, threads);
+    default: return -1.0;
+    }
+}
 
 // which: 0 std::sort, 1 std::sort(par_unseq), 2 __gnu_parallel::sort, 3 std::stable_sort
 // dtype codes: as the oracle (1..8).  Returns seconds spent sorting, < 0 on error.

@@ -50,6 +50,8 @@ def load():
     L.ts_oracle_dtype_size.argtypes = [i32]
     L.ts_cpu_sort.argtypes = [i32, i32, vp, vp, i64, i32, vp, vp]
     L.ts_cpu_sort.restype = ctypes.c_double
+    L.ts_cpu_segmented_sort.argtypes = [i32, i32, vp, vp, vp, i64, i32]
+    L.ts_cpu_segmented_sort.restype = ctypes.c_double
     _lib = L
     return L
 
@@ -137,3 +139,14 @@ def merge_ranks(k1, i1, k2, i2):
     out = np.empty(a.size + b.size, dtype=a.dtype)
     assert L.ts_oracle_from_ord(DTYPES[a.dtype], _p(ko), _p(out), out.size) == 0
     return out, io
+
+
+def cpu_segmented_sort(par, keys, idx, offsets, threads=0):
+    """seconds of a per-segment std::sort loop (par: under omp parallel for)"""
+    L = load()
+    k = np.ascontiguousarray(keys)
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    ix = None if idx is None else np.ascontiguousarray(idx, dtype=np.int32)
+    sec = L.ts_cpu_segmented_sort(int(par), DTYPES[k.dtype], _p(k), _p(ix), _p(off), off.size - 1, threads)
+    assert sec >= 0
+    return sec

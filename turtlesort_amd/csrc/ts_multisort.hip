@@ -21,7 +21,7 @@ namespace ts {
 // `per_wg` consecutive arrays, each in its own power-of-two region of
 // n_eff = next_pow2(arr_size) slots (per_wg * n_eff <= NT * ITEMS): short
 // arrays are packed so that every lane has keys to sort.
-template <typename U, bool FL, bool IDX, int NT, int ITEMS>
+template <typename U, bool FL, bool IDX, int NT, int ITEMS, bool PACK>
 __global__ void __launch_bounds__(NT) ms_kernel(const U *__restrict__ in, U *__restrict__ out,
                                                  const uint32_t *__restrict__ iin, uint32_t *__restrict__ iout,
                                                  uint32_t arr_size, uint32_t per_wg, uint64_t num_arrays, U flip)
@@ -29,6 +29,39 @@ __global__ void __launch_bounds__(NT) ms_kernel(const U *__restrict__ in, U *__r
     constexpr uint32_t CAPA = (uint32_t)NT * ITEMS;
     __shared__ U sk[padded_len<ITEMS>(CAPA)];
     __shared__ uint32_t si[IDX ? padded_len<ITEMS>(CAPA) : 1];
+    if constexpr (!PACK) {
+        // one array per workgroup
+        const size_t base = (size_t)blockIdx.x * arr_size;
+        uint32_t n_eff = next_pow2_dev(arr_size);
+        if (n_eff < (uint32_t)ITEMS) n_eff = ITEMS;
+        U kv[ITEMS];
+        uint32_t iv[ITEMS];
+#pragma unroll
+        for (int r = 0; r < ITEMS; r++) {
+            const uint32_t i = threadIdx.x + (uint32_t)r * NT;
+            kv[r] = i < arr_size ? in[base + i] : (U)0;
+            iv[r] = IDX && iin && i < arr_size ? iin[base + i] : i;
+        }
+#pragma unroll
+        for (int r = 0; r < ITEMS; r++) {
+            const uint32_t i = threadIdx.x + (uint32_t)r * NT;
+            if (i < n_eff) {
+                sk[pidx<ITEMS>(i)] = i < arr_size ? to_ord<U, FL>(kv[r], flip) : key_max<U>();
+                if (IDX) si[pidx<ITEMS>(i)] = iv[r];
+            }
+        }
+        block_sync();
+        block_sort<U, IDX, NT, ITEMS>(sk, si, n_eff, n_eff);
+#pragma unroll
+        for (int r = 0; r < ITEMS; r++) {
+            const uint32_t i = threadIdx.x + (uint32_t)r * NT;
+            if (i < arr_size) {
+                out[base + i] = from_ord<U, FL>(sk[pidx<ITEMS>(i)], flip);
+                if (IDX) iout[base + i] = si[pidx<ITEMS>(i)];
+            }
+        }
+        return;
+    }
     const uint64_t arr0 = (uint64_t)blockIdx.x * per_wg;
     const uint32_t nar = num_arrays - arr0 < per_wg ? (uint32_t)(num_arrays - arr0) : per_wg;
     uint32_t n_eff = next_pow2_dev(arr_size);
@@ -234,17 +267,21 @@ static hipError_t ms_launch(uint64_t flip, uint32_t arr_size, const void *in, vo
             // cross-wave barriers; 16 keys per thread take one merge round
             // fewer than 8; 65,536 x 1024 int32 keys: 0.34 -> 0.29 ms)
             const uint32_t per = 1024u / ne;
-            hipLaunchKernelGGL((ms_kernel<U, FL, IDX, 64, 16>), dim3((uint32_t)((na + per - 1) / per)), dim3(64), 0, st,
-                               ii, oo, xi, xo, arr_size, per, na, f);
+            if (per > 1)
+                hipLaunchKernelGGL((ms_kernel<U, FL, IDX, 64, 16, true>), dim3((uint32_t)((na + per - 1) / per)),
+                                   dim3(64), 0, st, ii, oo, xi, xo, arr_size, per, na, f);
+            else
+                hipLaunchKernelGGL((ms_kernel<U, FL, IDX, 64, 16, false>), dim3((uint32_t)na), dim3(64), 0, st, ii, oo,
+                                   xi, xo, arr_size, 1u, na, f);
         } else if (ne <= 2048)
-            hipLaunchKernelGGL((ms_kernel<U, FL, IDX, 128, 16>), dim3((uint32_t)na), dim3(128), 0, st, ii, oo, xi,
-                               xo, arr_size, 1u, na, f);
+            hipLaunchKernelGGL((ms_kernel<U, FL, IDX, 128, 16, false>), dim3((uint32_t)na), dim3(128), 0, st, ii, oo,
+                               xi, xo, arr_size, 1u, na, f);
         else if (ne <= 4096)
-            hipLaunchKernelGGL((ms_kernel<U, FL, IDX, 256, 16>), dim3((uint32_t)na), dim3(256), 0, st, ii, oo, xi,
-                               xo, arr_size, 1u, na, f);
+            hipLaunchKernelGGL((ms_kernel<U, FL, IDX, 256, 16, false>), dim3((uint32_t)na), dim3(256), 0, st, ii, oo,
+                               xi, xo, arr_size, 1u, na, f);
         else if (ne <= 8192)
-            hipLaunchKernelGGL((ms_kernel<U, FL, IDX, 512, 16>), dim3((uint32_t)na), dim3(512), 0, st, ii, oo, xi,
-                               xo, arr_size, 1u, na, f);
+            hipLaunchKernelGGL((ms_kernel<U, FL, IDX, 512, 16, false>), dim3((uint32_t)na), dim3(512), 0, st, ii, oo,
+                               xi, xo, arr_size, 1u, na, f);
         else
             return hipErrorInvalidValue;
         hipError_t e = hipGetLastError();
