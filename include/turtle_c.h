@@ -339,6 +339,13 @@ const char *turtle_status_string(int status);
 /* Last error message recorded by this thread (empty string if none). */
 const char *turtle_last_error(void);
 int turtle_dtype_size(int dtype);
+/* Runs, once per device, the check of the hardware property the sort
+ * kernels' stable ranks rely on: the lanes of one wave's LDS atomic to the
+ * same word receive their return values in ascending lane order (measured
+ * on gfx950; every handle creation and the allocation-free sort run it
+ * too).  TURTLE_OK, or TURTLE_ERR_UNSUPPORTED if the property does not hold.
+ * No reference counterpart. */
+int turtle_check_device(void);
 /* Number of compute units of the current device (0 if no device). */
 int turtle_device_cus(void);
 

@@ -292,3 +292,9 @@ def test_full_size_64M_int32_idx(torch_dev):
     assert bool(torch.equal(kin[iout.long()], kout))
     assert bool((torch.bincount(iout.long(), minlength=n) == 1).all())
     sorter.close()
+
+
+def test_lds_atomic_lane_order(torch_dev):
+    """The device property the stable ranks rely on (ranks from one LDS
+    atomic per key equal ranks from wave64 match masks)."""
+    assert _lib.load().turtle_check_device() == _lib.OK

@@ -145,6 +145,7 @@ SIGNATURES = {
     "turtle_last_error": (ctypes.c_char_p, []),
     "turtle_dtype_size": (_i32, [_i32]),
     "turtle_device_cus": (_i32, []),
+    "turtle_check_device": (_i32, []),
 }
 
 _lib = None

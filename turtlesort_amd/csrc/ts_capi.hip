@@ -736,6 +736,25 @@ int stage_d2h(turtle_sorter *h, void *host, const void *dev, size_t bytes, hipSt
     return TURTLE_OK;
 }
 
+// The stable ranks of the sort kernels rely on the lane order of LDS
+// atomic return values (rank16_add); it is checked once per device.
+int ensure_rank_order(int dev)
+{
+    static std::mutex mu;
+    static int state[64] = {0};   // 0 unchecked, 1 holds, 2 does not hold
+    if (dev < 0 || dev >= 64) return fail(TURTLE_ERR_UNSUPPORTED, "device ordinal out of range");
+    std::lock_guard<std::mutex> lk(mu);
+    if (state[dev] == 0) {
+        bool ok = false;
+        HIPCHK(ts::check_lds_rank_order(&ok), "LDS atomic order check");
+        state[dev] = ok ? 1 : 2;
+    }
+    if (state[dev] == 2)
+        return fail(TURTLE_ERR_UNSUPPORTED,
+                    "LDS atomics on this device do not return in lane order; the stable ranking needs it");
+    return TURTLE_OK;
+}
+
 int alloc_buf(void **p, size_t bytes, const char *what)
 {
     HIPCHK(hipMalloc(p, bytes ? bytes : 16), what);
@@ -767,6 +786,13 @@ const char *turtle_status_string(int s)
 const char *turtle_last_error(void) { return g_last_error.c_str(); }
 
 int turtle_dtype_size(int dtype) { return dtype_bytes(dtype); }
+
+int turtle_check_device(void)
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev), "hipGetDevice");
+    return ensure_rank_order(dev);
+}
 
 int turtle_device_cus(void)
 {
@@ -802,6 +828,10 @@ int turtle_create(int dtype, int64_t max_n, int track_index, int compress, turtl
         return fail_hip(e, "no HIP device");
     }
     h->cus = pr.multiProcessorCount;
+    if ((rc = ensure_rank_order(h->device))) {
+        delete h;
+        return rc;
+    }
     const size_t kbytes = (size_t)max_n * kb, ibytes = (size_t)max_n * 4;
     if ((rc = alloc_buf(&h->kin, kbytes, "hipMalloc(keys in)")) || (rc = alloc_buf(&h->kout, kbytes, "hipMalloc(keys out)")) ||
         (rc = alloc_buf(&h->kt0, kbytes, "hipMalloc(keys tmp0)")) || (rc = alloc_buf(&h->kt1, kbytes, "hipMalloc(keys tmp1)"))) {
@@ -1482,6 +1512,8 @@ int na_device(int dev, NoAllocDevice **out)
     std::lock_guard<std::mutex> lk(g_na_init);
     NoAllocDevice &d = g_na[dev];
     if (!d.base) {
+        int rc = ensure_rank_order(dev);
+        if (rc) return rc;
         void *ptr = nullptr;
         HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
         int cus = 0;

@@ -45,6 +45,12 @@ uint32_t multisort_max_size(int kb);
 hipError_t merge_run(int kt, uint64_t flip, const void *a, const uint32_t *ia, uint64_t na, const void *b,
                      const uint32_t *ib, uint64_t nb, void *out, uint32_t *iout, hipStream_t st);
 
+// Checks on the current device that the lanes of one wave's LDS atomic to
+// the same word receive their return values in ascending lane order (the
+// stable ranks of the scatter, the radix LDS sort and the pivot rely on
+// it); synchronous (ts_util.hip).
+hipError_t check_lds_rank_order(bool *ok);
+
 // synthetic inputs (ts_util.hip)
 hipError_t fill_keys(int kb, void *keys, uint64_t n, uint64_t seed, int shift, hipStream_t st);
 hipError_t fill_iota(uint32_t *idx, uint64_t n, hipStream_t st, uint64_t period = 0);   // period > 0: i mod period

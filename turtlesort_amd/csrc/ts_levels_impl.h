@@ -521,20 +521,40 @@ struct LevelK {
                 pk[c] = e < tl ? b : (uint32_t)NB;
             }
         }
-        // rank: running per-wave bin counter (one wave owns wcnt[w], its LDS
-        // accesses complete in program order) + rank among the lanes of this
-        // step with the same bin
+        // rank: one LDS atomic on the wave's own bin counter (one wave owns
+        // wcnt[w]); its return value counts the wave's earlier keys of the
+        // bin in (item, lane) order, i.e. in element order (see rank16_add)
+        // An item whose valid lanes all hold one bin (sorted runs) takes its
+        // ranks with one atomic: 64 same-address atomics are serialised by
+        // the LDS.  Items are tested only when the wave's first or last item
+        // is such one, so that random keys pay two tests per tile.
+        auto uniform = [&](int c, uint32_t &b0, uint64_t &vm) {
+            const uint32_t e = (uint32_t)w * (TI * WAVE) + c * WAVE + ln;
+            const bool valid = (FULL || e < tl) && pk[c] < (uint32_t)NB;
+            b0 = __builtin_amdgcn_readfirstlane(pk[c]);
+            vm = ballot(valid);
+            return vm != 0 && ballot(valid && pk[c] == b0) == vm;
+        };
+        uint32_t bu;
+        uint64_t mu;
+        const bool runs = uniform(0, bu, mu) || uniform(TI - 1, bu, mu);   // wave-uniform
 #pragma unroll
         for (int c = 0; c < TI; c++) {
             const uint32_t e = (uint32_t)w * (TI * WAVE) + c * WAVE + ln;
             const bool valid = (FULL || e < tl) && pk[c] < (uint32_t)NB;
             bad |= (FULL || e < tl) && pk[c] >= (uint32_t)NB;
             const uint32_t bin = valid ? pk[c] : (uint32_t)NB;
-            const uint64_t peers = match_peers<BITS>(bin, valid);
-            const uint32_t r = mask_rank(peers);
-            const uint32_t old = sh.wcnt[w][bin];
-            if (valid && r == 0) sh.wcnt[w][bin] = old + (uint32_t)__popcll(peers);
-            pk[c] = bin | ((old + r) << 8);
+            uint32_t old = 0, b0;
+            uint64_t vm;
+            if (runs && uniform(c, b0, vm)) {   // wave-uniform
+                const uint32_t first = (uint32_t)__builtin_ctzll(vm);
+                uint32_t o0 = 0;
+                if (ln == first) o0 = atomicAdd(&sh.wcnt[w][b0], (uint32_t)__popcll(vm));
+                old = __builtin_amdgcn_readlane(o0, first) + mask_rank(vm);
+            } else if (valid) {
+                old = atomicAdd(&sh.wcnt[w][bin], 1u);   // rank in (item, lane) order: see rank16_add
+            }
+            pk[c] = bin | (old << 8);
         }
         // the first half of the indices is loaded now (they are not needed
         // for the ranking; the barrier and the offset scan cover the
@@ -840,7 +860,7 @@ struct LevelK {
         // the step's tasks: task of wave w (NW when idle), first wave and
         // waves of task t (tasks hold consecutive waves; tw0[nk] = waves used)
         uint32_t wq[NW], tw0[NW + 1], tkw[NW];
-        uint16_t cnt[NW][NBIN];   // per-wave digit counts, then the digits' first slots
+        alignas(4) uint16_t cnt[NW][NBIN];   // per-wave digit counts (rank16_add), then the digits' first slots
         U wmin[NW], wmax[NW];
         U tmin[NW];
         uint32_t red[16];
@@ -848,7 +868,7 @@ struct LevelK {
     };
 
     struct PivotShared {
-        uint16_t cnt[NW][NBIN];       // per-wave digit counts, then the digits' first positions
+        alignas(4) uint16_t cnt[NW][NBIN];   // per-wave digit counts (rank16_add), then the digits' first positions
         U buf[NW][C::SAMPLE];         // per-wave sample
         uint32_t ntl[NW];             // tiles of each wave's segment in this step
         uint32_t tbase, done;         // the step's first tile; segments done
@@ -992,16 +1012,32 @@ struct LevelK {
             if (act) {
                 for (uint32_t d = l; d < NBIN; d += WAVE) rs.cnt[w][d] = 0;
                 wave_sync();
+                // after a pass the task's keys are at its first len positions
+                auto digit = [&](int i) {
+                    const bool real = j0 + (uint32_t)i * WAVE < tr.len;
+                    return real ? (uint32_t)((U)(k[i] - tmin) >> sh) & (NBIN - 1) : NBIN - 1;
+                };
+                // an item whose lanes share one digit (runs of sorted keys)
+                // takes its ranks with one atomic (64 same-address atomics
+                // are serialised by the LDS); tested per item only when the
+                // wave's first or last item is such one
+                auto same = [&](uint32_t d) {
+                    const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+                    return ballot(d == d0) == ~0ull;
+                };
+                const bool runs = same(digit(0)) || same(digit(SI - 1));   // wave-uniform
 #pragma unroll
                 for (int i = 0; i < SI; i++) {
-                    // after a pass the task's keys are at its first len positions
-                    const bool real = j0 + (uint32_t)i * WAVE < tr.len;
-                    const uint32_t d = real ? (uint32_t)((U)(k[i] - tmin) >> sh) & (NBIN - 1) : NBIN - 1;
-                    const uint64_t peers = match_peers<RW>(d, true);
-                    const uint32_t r = mask_rank(peers);
-                    const uint32_t old = rs.cnt[w][d];
-                    if (r == 0) rs.cnt[w][d] = (uint16_t)(old + (uint32_t)__popcll(peers));
-                    dr[i] = d | ((old + r) << 16);
+                    const uint32_t d = digit(i);
+                    uint32_t r;
+                    if (runs && same(d)) {   // wave-uniform
+                        uint32_t o0 = 0;
+                        if (l == 0) o0 = rank16_add((uint32_t *)rs.cnt[w], d, (uint32_t)WAVE);
+                        r = __builtin_amdgcn_readlane(o0, 0) + l;
+                    } else {
+                        r = rank16_add((uint32_t *)rs.cnt[w], d, 1u);
+                    }
+                    dr[i] = d | (r << 16);
                 }
             }
             block_sync();
@@ -1329,11 +1365,7 @@ struct LevelK {
                             // after a pass the sample's keys are at its first m positions
                             const bool real = (uint32_t)i * WAVE + l < m;
                             const uint32_t d = real ? (uint32_t)((U)(k[i] - mn) >> sh) & (NBIN - 1) : NBIN - 1;
-                            const uint64_t peers = match_peers<RW>(d, true);
-                            const uint32_t r = mask_rank(peers);
-                            const uint32_t old = cn[d];
-                            if (r == 0) cn[d] = (uint16_t)(old + (uint32_t)__popcll(peers));
-                            dr[i] = d | ((old + r) << 16);
+                            dr[i] = d | (rank16_add((uint32_t *)cn, d, 1u) << 16);
                         }
                         wave_sync();
                         // exclusive scan of the NBIN digit counts: lane l owns NBIN/WAVE digits

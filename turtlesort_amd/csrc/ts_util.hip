@@ -1,4 +1,5 @@
-// ts_util.hip -- synthetic input generation on the device.
+// ts_util.hip -- synthetic input generation on the device, and the check
+// of the LDS atomic ordering that the stable ranks rely on.
 // splitmix64 is counter based: draw i (0-based) of a stream seeded with s is
 // mix(s + (i+1) * 0x9e3779b97f4a7c15), so every element is generated
 // independently and matches the sequential host generator draw for draw.
@@ -58,6 +59,58 @@ hipError_t fill_iota(uint32_t *idx, uint64_t n, hipStream_t st, uint64_t period)
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(fill_iota_kernel, dim3(grid_for(n)), dim3(256), 0, st, idx, n, period);
     return hipGetLastError();
+}
+
+// ---- LDS atomic lane order ------------------------------------------------------
+// The scatter, the radix LDS sort and the pivot rank keys with one LDS
+// atomic each (rank16_add), which is stable only if the lanes of one wave's
+// atomic to the same word get their return values in ascending lane order.
+// This kernel compares those ranks with ranks computed from wave64 match
+// masks (the number of lower lanes with the same counter plus the
+// counter's value before the instruction) over patterns of 1, 2, 4, 16 and
+// 512 distinct counters, 16-bit halves included, and sets the flag on any
+// difference.
+__device__ uint32_t g_rank_order_bad;
+
+__global__ void __launch_bounds__(512) k_rank_order_check()
+{
+    constexpr uint32_t NC = 512;
+    __shared__ uint32_t words[8][NC / 2];   // atomic counters, two per word
+    __shared__ uint32_t ref[8][NC];         // counts from match masks
+    const uint32_t w = threadIdx.x / WAVE, l = lane();
+    for (uint32_t i = l; i < NC; i += WAVE) {
+        ref[w][i] = 0;
+        if (i < NC / 2) words[w][i] = 0;
+    }
+    wave_sync();
+    uint32_t bad = 0;
+    for (uint32_t r = 0; r < 64; r++) {
+        const uint32_t mask = (r % 5 == 0) ? 0u : (r % 5 == 1) ? 1u : (r % 5 == 2) ? 3u : (r % 5 == 3) ? 15u : NC - 1;
+        const uint32_t c = (uint32_t)splitmix_at(0x51ab1e00ull + blockIdx.x, (uint64_t)r * 512 + threadIdx.x) & mask;
+        const uint32_t got = rank16_add(words[w], c, 1u);
+        const uint64_t peers = match_peers<9>(c, true);
+        const uint32_t rk = mask_rank(peers);
+        const uint32_t old = ref[w][c];
+        wave_sync();
+        if (rk == 0) ref[w][c] = old + (uint32_t)__popcll(peers);
+        wave_sync();
+        bad |= got != old + rk ? 1u : 0u;
+    }
+    if (bad) atomic_or(&g_rank_order_bad, 1u);
+}
+
+hipError_t check_lds_rank_order(bool *ok)
+{
+    *ok = false;
+    const uint32_t zero = 0;
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(g_rank_order_bad), &zero, sizeof zero);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_rank_order_check, dim3(256), dim3(512), 0, 0);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    uint32_t bad = 1;
+    if ((e = hipMemcpyFromSymbol(&bad, HIP_SYMBOL(g_rank_order_bad), sizeof bad)) != hipSuccess) return e;
+    *ok = bad == 0;
+    return hipSuccess;
 }
 
 }  // namespace ts
