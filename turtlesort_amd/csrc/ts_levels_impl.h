@@ -505,6 +505,14 @@ struct LevelK {
         U x[TI];
         uint32_t ix[TI];
         uint32_t pk[TI];   // bin | rank inside the wave's bin << 8
+        // wave 0 needs the tile's destination prefixes after the ranking:
+        // their loads are issued first, so that their latency is covered
+        uint32_t pre_bin = 0, pre_seg = 0;
+        if (w == 0 && ln < (uint32_t)NB) {
+            const size_t e0 = (size_t)tb * NB;
+            pre_bin = p.tpre[e0 + (size_t)ln * ntl + t];
+            pre_seg = p.tpre[e0];
+        }
         // keys and the bins the count pass stored for them
         uint32_t bw[TI / 4];
 #pragma unroll
@@ -581,9 +589,7 @@ struct LevelK {
                     sh.woff[i][l] = run;
                     run += sh.wcnt[i][l];
                 }
-                const size_t e0 = (size_t)tb * NB;
-                const uint32_t rel = p.tpre[e0 + (size_t)l * ntl + t] - p.tpre[e0];
-                sh.dbase[l] = begin + rel - start;
+                sh.dbase[l] = begin + (pre_bin - pre_seg) - start;
             }
         }
         block_sync();
