@@ -7,11 +7,11 @@
 // turtle-sort.cuh:192-243), i.e. a stable merge in which run 1 goes first on
 // equal keys, at log2(n) dependent global loads per element.
 //
-// Here a workgroup produces TILE consecutive outputs.  Its two waves find
-// where the tile's first and last diagonals cross the merge path (the
-// number of run-1 elements among the first d outputs) with a 64-way search:
-// each round every lane tests one evenly spaced candidate and a ballot
-// keeps the interval between the last passing and the first failing
+// Here a workgroup produces TILE consecutive outputs.  Where the tile
+// boundaries cross the merge path (the number of run-1 elements among the
+// first d outputs) is found first, one wave per boundary, with a 64-way
+// search: each round every lane tests one evenly spaced candidate and a
+// ballot keeps the interval between the last passing and the first failing
 // candidate, so a run of 2^26 keys takes 5 rounds of loads instead of 26.
 // The tile's slices of both runs are loaded into LDS with coalesced loads,
 // every thread merges ITEMS outputs from its own diagonal (a binary search
@@ -62,6 +62,27 @@ __device__ __forceinline__ uint64_t corank_wave(const U *a, uint64_t na, const U
     return lo;
 }
 
+// The crossings of the tile boundaries, found before the merge by one wave
+// each, all at once, so that no merge workgroup starts with a chain of
+// dependent loads.  The crossing of boundary k (diagonal k*TILE) is kept in
+// the output itself, where only the two workgroups that need it read it
+// before they write anything: the first 8 bytes of tile k and the last 8
+// bytes of tile k-1 (a tile's keys take TILE * sizeof(U) >= 4096 bytes).
+template <typename U, bool FL>
+__global__ void __launch_bounds__(256) k_merge_corank(const U *__restrict__ a, uint64_t na, const U *__restrict__ b,
+                                                      uint64_t nb, unsigned char *__restrict__ out, uint64_t ntiles,
+                                                      U flip)
+{
+    const uint64_t k = 1 + (uint64_t)blockIdx.x * (256 / WAVE) + threadIdx.x / WAVE;   // boundary 1 .. ntiles-1
+    if (k >= ntiles) return;   // wave-uniform
+    const uint64_t d = k * MERGE_TILE;
+    const uint64_t c = corank_wave<U, FL>(a, na, b, nb, d, flip);
+    if (lane() == 0) {
+        *(uint64_t *)(out + d * sizeof(U)) = c;
+        *(uint64_t *)(out + d * sizeof(U) - 8) = c;
+    }
+}
+
 template <typename U, bool FL, bool IDX>
 __global__ void __launch_bounds__(MERGE_NT) k_merge(const U *__restrict__ a, const uint32_t *__restrict__ ia, uint64_t na,
                                                     const U *__restrict__ b, const uint32_t *__restrict__ ib, uint64_t nb,
@@ -76,10 +97,12 @@ __global__ void __launch_bounds__(MERGE_NT) k_merge(const U *__restrict__ a, con
     const uint64_t d0 = (uint64_t)blockIdx.x * T;
     const uint64_t d1 = d0 + T < ntot ? d0 + T : ntot;
     const uint32_t tid = threadIdx.x;
-    const uint32_t w = tid / WAVE;
-    if (w < 2) {
-        const uint64_t c = corank_wave<U, FL>(a, na, b, nb, w ? d1 : d0, flip);
-        if (lane() == 0) cor[w] = c;
+    if (tid == 0) {
+        // the crossings k_merge_corank left in this tile (read before the
+        // tile is written)
+        const unsigned char *ob = (const unsigned char *)out;
+        cor[0] = d0 == 0 ? 0ull : *(const uint64_t *)(ob + d0 * sizeof(U));
+        cor[1] = d1 == ntot ? na : *(const uint64_t *)(ob + d1 * sizeof(U) - 8);
     }
     block_sync();
     const uint64_t a0 = cor[0], a1 = cor[1];
@@ -166,6 +189,12 @@ static hipError_t merge_launch(uint64_t flip, const void *a, const uint32_t *ia,
     const uint64_t grid = (n + MERGE_TILE - 1) / MERGE_TILE;
     if (grid == 0) return hipSuccess;
     if (grid > 0x7fffffffull) return hipErrorInvalidValue;
+    static_assert(MERGE_TILE * sizeof(U) >= 16, "two crossings fit in a tile");
+    if (grid > 1) {
+        const uint64_t nb_wg = (grid - 1 + 3) / 4;
+        hipLaunchKernelGGL((k_merge_corank<U, FL>), dim3((uint32_t)nb_wg), dim3(256), 0, st, (const U *)a, na,
+                           (const U *)b, nb, (unsigned char *)out, grid, (U)flip);
+    }
     if (iout)
         hipLaunchKernelGGL((k_merge<U, FL, true>), dim3((uint32_t)grid), dim3(MERGE_NT), 0, st, (const U *)a, ia, na,
                            (const U *)b, ib, nb, (U *)out, iout, (U)flip);
