@@ -387,6 +387,36 @@ def bench_serving(seconds, rate):
     return out
 
 
+P99_BOUND_MS = 50.0
+
+
+def bench_saturation(seconds=8.0, clients=(8, 32, 128)):
+    """closed-loop saturation of the request loop: C clients each submit a
+    request of the serving distribution and wait for it, again and again;
+    capacity = the highest throughput whose p99 latency stays under
+    P99_BOUND_MS"""
+    import subprocess
+    exe = os.path.join(ROOT, "examples", "build", "serve_load")
+    if not os.path.exists(exe):
+        return {"error": f"{exe} not built"}
+    runs = []
+    for c in clients:
+        cmd = [exe, "--closed", str(c), "--seconds", str(seconds), "--pool-gb", "64", "--slots", "4",
+               "--slot-keys", str(16 << 20)]
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=seconds * 4 + 120)
+        lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
+        if r.returncode != 0 or not lines:
+            runs.append({"clients": c, "error": f"exit {r.returncode}", "stderr": r.stderr[-300:]})
+            continue
+        runs.append(json.loads(lines[-1]))
+    ok = [x for x in runs if "error" not in x and x["p99_ms"] <= P99_BOUND_MS and x["errors"] == 0]
+    best = max(ok, key=lambda x: x["Gkeys_s"]) if ok else None
+    return {"workload": "closed loop, sizes log-uniform [1K, 4M], dtype uniform {uint8,int16,int32,int64}, "
+                        "index on 50%, 4 slots of 16M keys per type, 64 GB pool",
+            "p99_bound_ms": P99_BOUND_MS, "capacity_Gkeys_s": best["Gkeys_s"] if best else None,
+            "capacity_clients": best["clients"] if best else None, "runs": runs}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -536,6 +566,7 @@ def main():
 
     if not args.no_serve:
         out["serving"] = bench_serving(args.serve_seconds, args.serve_rate)
+        out["serving_saturation"] = bench_saturation()
 
     if not args.no_cpu:
         from oracle import oracle

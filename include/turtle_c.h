@@ -298,6 +298,12 @@ typedef struct turtle_server_config {
                                   (0: from the budget, at most 32M) */
     int32_t slots_per_dtype;   /* batches of one key type in flight at once (0: 2) */
     int32_t dtype_mask;        /* bit (1 << dtype) for every served key type (0: all) */
+    int64_t inflight_bytes;    /* device bytes the requests being sorted may use at
+                                  once (0: pool_bytes).  A request of n keys of K
+                                  bytes is charged n * (4 K + 17) bytes (input and
+                                  output keys and indices, the engine's ping-pong
+                                  keys and indices, per-key bins) while its batch
+                                  runs; it joins a batch only when its bytes fit. */
 } turtle_server_config;
 
 typedef struct turtle_server_stats {
@@ -305,21 +311,37 @@ typedef struct turtle_server_stats {
     uint64_t pool_bytes, slot_keys;
     double busy_seconds;           /* summed batch time of all slots */
     uint32_t slots, in_flight_batches, max_in_flight_batches, pad;
+    uint64_t inflight_bytes;       /* the budget of requests being sorted */
+    uint64_t bytes_in_flight, max_bytes_in_flight;
 } turtle_server_stats;
+
+/* What happened to one request (turtle_server_wait_info). */
+typedef struct turtle_request_info {
+    double latency_seconds;   /* submission to completion */
+    double queued_seconds;    /* submission to the start of its batch */
+    double batch_seconds;     /* its batch's run: copies in, sort, copies out */
+    int64_t batch_keys;       /* keys of its batch */
+    int32_t batch_requests;   /* requests of its batch */
+    int32_t status;           /* the request's status */
+} turtle_request_info;
 
 /* Allocates every slot (device buffers, pinned host buffers) and starts one
  * worker thread per slot.  TURTLE_ERR_BUDGET if the slots do not fit the
  * pool. */
 int turtle_server_create(const turtle_server_config *cfg, turtle_server_handle *out);
 /* Queues a request (returns at once with its ticket).  TURTLE_ERR_BUDGET if
- * n exceeds slot_keys; TURTLE_ERR_DTYPE if the type is not served.  keys
- * (and idx) must stay alive and untouched until the ticket is waited for. */
+ * n exceeds slot_keys or its bytes exceed inflight_bytes; TURTLE_ERR_DTYPE
+ * if the type is not served.  keys (and idx) must stay alive and untouched
+ * until the ticket is waited for. */
 int turtle_server_submit(turtle_server_handle s, int dtype, void *keys, int32_t *idx, int64_t n,
                          uint64_t *ticket);
 /* Blocks until the request is sorted; *latency_seconds = completion time -
  * submission time.  Returns the request's status.  A ticket is waited for
  * once. */
 int turtle_server_wait(turtle_server_handle s, uint64_t ticket, double *latency_seconds);
+/* As turtle_server_wait, and fills *info (may be NULL) with the request's
+ * queueing and batch figures. */
+int turtle_server_wait_info(turtle_server_handle s, uint64_t ticket, turtle_request_info *info);
 /* *done = 1 if the request has finished (then wait returns at once). */
 int turtle_server_test(turtle_server_handle s, uint64_t ticket, int *done);
 int turtle_server_get_stats(turtle_server_handle s, turtle_server_stats *out);

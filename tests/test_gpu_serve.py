@@ -112,3 +112,34 @@ def test_server_float_requests(orc, torch_dev):
         np.testing.assert_array_equal(k.view(ui), ok.view(ui))
         if idx is not None:
             np.testing.assert_array_equal(x, np.argsort(orc.order(keys), kind="stable"))
+
+
+def test_inflight_budget_admission(orc, torch_dev):
+    """Requests are charged n * (4 K + 17) device bytes while their batch
+    runs and join a batch only when those bytes fit the in-flight budget:
+    with room for one 1M-key int32 request, four slots still sort one
+    request at a time; a request larger than the budget is refused."""
+    per = 1_000_000 * (4 * 4 + 17)
+    srv = SortServer(pool_bytes=8 << 30, slot_keys=1 << 21, slots_per_dtype=4, dtypes=[np.int32],
+                     inflight_bytes=per + per // 2)
+    reqs = []
+    for i in range(6):
+        keys = wl.gen_keys(np.int32, 1_000_000, 77 + i, 0)
+        k = keys.copy()
+        reqs.append((keys, k, srv.submit(k)))
+    infos = []
+    for keys, k, t in reqs:
+        infos.append(srv.wait_info(t))
+        ok, _ = orc.turtle_sort(keys)
+        np.testing.assert_array_equal(k, ok)
+    st = srv.stats()
+    assert st["max_bytes_in_flight"] <= per + per // 2
+    assert st["max_in_flight_batches"] == 1
+    assert st["bytes_in_flight"] == 0 and st["completed"] == 6
+    assert all(i["batch_requests"] == 1 and i["status"] == 0 for i in infos)
+    # the later requests waited for the budget
+    assert max(i["queued_seconds"] for i in infos) > 0
+    with pytest.raises(TurtleError) as e:
+        srv.submit(np.zeros(2_000_000, dtype=np.int32))
+    assert e.value.status == 10   # TURTLE_ERR_BUDGET
+    srv.close()

@@ -85,6 +85,7 @@ class ServerConfig(ctypes.Structure):
         ("slot_keys", ctypes.c_int64),
         ("slots_per_dtype", ctypes.c_int32),
         ("dtype_mask", ctypes.c_int32),
+        ("inflight_bytes", ctypes.c_int64),
     ]
 
 
@@ -96,10 +97,23 @@ class ServerStats(ctypes.Structure):
         ("busy_seconds", ctypes.c_double),
         ("slots", ctypes.c_uint32), ("in_flight_batches", ctypes.c_uint32),
         ("max_in_flight_batches", ctypes.c_uint32), ("pad", ctypes.c_uint32),
+        ("inflight_bytes", ctypes.c_uint64), ("bytes_in_flight", ctypes.c_uint64),
+        ("max_bytes_in_flight", ctypes.c_uint64),
     ]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [
+        ("latency_seconds", ctypes.c_double), ("queued_seconds", ctypes.c_double),
+        ("batch_seconds", ctypes.c_double), ("batch_keys", ctypes.c_int64),
+        ("batch_requests", ctypes.c_int32), ("status", ctypes.c_int32),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 _vp = ctypes.c_void_p
@@ -136,6 +150,7 @@ SIGNATURES = {
     "turtle_server_create": (_i32, [ctypes.POINTER(ServerConfig), ctypes.POINTER(_vp)]),
     "turtle_server_submit": (_i32, [_vp, _i32, _vp, _vp, _i64, ctypes.POINTER(_u64)]),
     "turtle_server_wait": (_i32, [_vp, _u64, ctypes.POINTER(ctypes.c_double)]),
+    "turtle_server_wait_info": (_i32, [_vp, _u64, ctypes.POINTER(RequestInfo)]),
     "turtle_server_test": (_i32, [_vp, _u64, ctypes.POINTER(_i32)]),
     "turtle_server_get_stats": (_i32, [_vp, ctypes.POINTER(ServerStats)]),
     "turtle_server_destroy": (_i32, [_vp]),

@@ -378,12 +378,13 @@ class SortServer:
     returns a ticket; wait(ticket) blocks until it is sorted and returns the
     latency in seconds.  The arrays must stay alive until waited for."""
 
-    def __init__(self, pool_bytes=0, slot_keys=0, slots_per_dtype=0, dtypes=None):
+    def __init__(self, pool_bytes=0, slot_keys=0, slots_per_dtype=0, dtypes=None, inflight_bytes=0):
         self._lib = _lib.load()
         cfg = _lib.ServerConfig()
         cfg.pool_bytes = int(pool_bytes)
         cfg.slot_keys = int(slot_keys)
         cfg.slots_per_dtype = int(slots_per_dtype)
+        cfg.inflight_bytes = int(inflight_bytes)
         cfg.dtype_mask = 0 if dtypes is None else sum(1 << dtype_code(d) for d in dtypes)
         h = ctypes.c_void_p()
         _lib.check(self._lib.turtle_server_create(ctypes.byref(cfg), ctypes.byref(h)))
@@ -409,6 +410,15 @@ class SortServer:
         finally:
             self._arrays.pop(ticket, None)
         return lat.value
+
+    def wait_info(self, ticket):
+        """as wait(), returning the request's queueing and batch figures"""
+        info = _lib.RequestInfo()
+        try:
+            _lib.check(self._lib.turtle_server_wait_info(self._h, ctypes.c_uint64(ticket), ctypes.byref(info)))
+        finally:
+            self._arrays.pop(ticket, None)
+        return info.as_dict()
 
     def stats(self):
         st = _lib.ServerStats()

@@ -6,8 +6,10 @@
 // created: per served key type, `slots_per_dtype` sorter handles (each with
 // its own HIP stream and key/index buffers for `slot_keys` keys) plus a
 // pinned host buffer of the same capacity.  Every slot has a worker thread.
-// A worker that is free takes, in arrival order, every pending request of
-// its key type that still fits the slot, copies them back to back into its
+// Each request is charged its device bytes (request_bytes) against the
+// in-flight budget while its batch runs.  A worker that is free takes, in
+// arrival order, every pending request of its key type that still fits the
+// slot and the budget, copies them back to back into its
 // pinned buffer, sorts them as the segments of one engine run
 // (turtle_segmented_sort_device), copies the result back into each
 // request's arrays and signals completion.  Requests arriving while other
@@ -42,13 +44,19 @@ double now_s()
 
 int key_bytes(int dtype) { return turtle_dtype_size(dtype); }
 
+// device bytes of a request while its batch runs: input and output keys and
+// indices of the slot, the engine's ping-pong keys and indices, per-key bins
+int64_t request_bytes(int dtype, int64_t n) { return n * (4 * (int64_t)key_bytes(dtype) + 17); }
+
 struct Request {
     uint64_t ticket = 0;
     int dtype = 0;
     void *keys = nullptr;
     int32_t *idx = nullptr;
     int64_t n = 0;
-    double t_submit = 0.0, t_done = 0.0;
+    double t_submit = 0.0, t_start = 0.0, t_done = 0.0;
+    int64_t batch_keys = 0;
+    int32_t batch_requests = 0;
     int status = TURTLE_OK;
     bool done = false;
 };
@@ -79,6 +87,7 @@ struct turtle_server {
     int device = 0;
     turtle_server_stats st{};
     int64_t max_keys[TURTLE_FLOAT64 + 1] = {0};
+    int64_t inflight = 0;           // bytes of the requests being sorted
 };
 
 namespace {
@@ -145,30 +154,51 @@ void worker_loop(turtle_server *sv, Slot *s)
     (void)hipSetDevice(sv->device);
     for (;;) {
         std::vector<std::shared_ptr<Request>> batch;
+        int64_t batch_bytes = 0, batch_keys = 0;
+        double t_start = 0.0;
         {
             std::unique_lock<std::mutex> lk(sv->mu);
             auto &q = sv->pending[s->dtype];
-            sv->cv_work.wait(lk, [&] { return sv->stop || !q.empty(); });
+            const int64_t budget = sv->cfg.inflight_bytes;
+            // the first pending request must fit the bytes left in flight
+            sv->cv_work.wait(lk, [&] {
+                return (sv->stop && q.empty()) ||
+                       (!q.empty() && sv->inflight + request_bytes(s->dtype, q.front()->n) <= budget);
+            });
             if (sv->stop && q.empty()) return;
-            // every pending request of this type that still fits, in order
+            // every pending request of this type that still fits the slot
+            // and the budget, in order
             int64_t room = sv->cfg.slot_keys;
-            while (!q.empty() && q.front()->n <= room) {
+            while (!q.empty() && q.front()->n <= room &&
+                   sv->inflight + batch_bytes + request_bytes(s->dtype, q.front()->n) <= budget) {
                 room -= q.front()->n;
+                batch_bytes += request_bytes(s->dtype, q.front()->n);
+                batch_keys += q.front()->n;
                 batch.push_back(q.front());
                 q.pop_front();
+            }
+            sv->inflight += batch_bytes;
+            sv->st.bytes_in_flight = (uint64_t)sv->inflight;
+            if (sv->st.bytes_in_flight > sv->st.max_bytes_in_flight) sv->st.max_bytes_in_flight = sv->st.bytes_in_flight;
+            t_start = now_s();
+            for (auto &r : batch) {
+                r->t_start = t_start;
+                r->batch_keys = batch_keys;
+                r->batch_requests = (int32_t)batch.size();
             }
             sv->st.batches++;
             sv->st.in_flight_batches++;
             if (sv->st.in_flight_batches > sv->st.max_in_flight_batches)
                 sv->st.max_in_flight_batches = sv->st.in_flight_batches;
         }
-        const double t0 = now_s();
         const int rc = run_batch(sv, s, batch);
         const double t1 = now_s();
         {
             std::lock_guard<std::mutex> lk(sv->mu);
             sv->st.in_flight_batches--;
-            sv->st.busy_seconds += t1 - t0;
+            sv->st.busy_seconds += t1 - t_start;
+            sv->inflight -= batch_bytes;
+            sv->st.bytes_in_flight = (uint64_t)sv->inflight;
             for (auto &r : batch) {
                 r->status = rc;
                 r->done = true;
@@ -179,6 +209,7 @@ void worker_loop(turtle_server *sv, Slot *s)
             }
         }
         sv->cv_done.notify_all();
+        sv->cv_work.notify_all();   // bytes were freed: other workers may admit now
     }
 }
 
@@ -207,6 +238,7 @@ int turtle_server_create(const turtle_server_config *cfg, turtle_server_handle *
     }
     if (c.slot_keys < 1 || c.slot_keys >= ((int64_t)1 << 31)) return TURTLE_ERR_INVALID;
     if (c.slot_keys * per_key_sum > c.pool_bytes) return TURTLE_ERR_BUDGET;
+    if (c.inflight_bytes <= 0) c.inflight_bytes = c.pool_bytes;
     turtle_server *sv = new turtle_server();
     sv->cfg = c;
     (void)hipGetDevice(&sv->device);
@@ -232,6 +264,30 @@ int turtle_server_create(const turtle_server_config *cfg, turtle_server_handle *
                 slot_free(s.get());
                 break;
             }
+            // one warm-up run per engine form (with and without an index,
+            // at a small and a large size, which take different tunings):
+            // the first launch of a kernel pays its loading, which would
+            // otherwise land in the first requests' latency
+            if (hipMemsetAsync(s->dkin, 0, kbytes, s->stream) != hipSuccess ||
+                hipMemsetAsync(s->diin, 0, ibytes, s->stream) != hipSuccess) {
+                slot_free(s.get());
+                rc = TURTLE_ERR_DEVICE;
+                break;
+            }
+            const int64_t big = c.slot_keys < ((int64_t)4 << 20) ? c.slot_keys : ((int64_t)4 << 20);
+            for (int64_t wn : {(int64_t)65536, big}) {
+                if (wn > c.slot_keys) continue;
+                const int64_t off[2] = {0, wn};
+                for (int with_idx = 0; with_idx < 2 && rc == TURTLE_OK; with_idx++) {
+                    rc = turtle_segmented_sort_device(s->h, s->dkin, with_idx ? s->diin : nullptr, s->dkout,
+                                                      with_idx ? s->diout : nullptr, off, 1, s->stream);
+                    if (rc == TURTLE_OK) rc = turtle_synchronize(s->h, nullptr);
+                }
+            }
+            if (rc != TURTLE_OK) {
+                slot_free(s.get());
+                break;
+            }
             sv->slots.push_back(std::move(s));
         }
     }
@@ -241,6 +297,7 @@ int turtle_server_create(const turtle_server_config *cfg, turtle_server_handle *
         return rc;
     }
     sv->st.pool_bytes = (uint64_t)c.pool_bytes;
+    sv->st.inflight_bytes = (uint64_t)c.inflight_bytes;
     sv->st.slot_keys = (uint64_t)c.slot_keys;
     sv->st.slots = (uint32_t)sv->slots.size();
     for (auto &s : sv->slots) s->worker = std::thread(worker_loop, sv, s.get());
@@ -263,7 +320,7 @@ int turtle_server_submit(turtle_server_handle sv, int dtype, void *keys, int32_t
     {
         std::lock_guard<std::mutex> lk(sv->mu);
         if (sv->max_keys[dtype] == 0) return TURTLE_ERR_DTYPE;   // type not served
-        if (n > sv->max_keys[dtype]) {
+        if (n > sv->max_keys[dtype] || request_bytes(dtype, n) > sv->cfg.inflight_bytes) {
             sv->st.rejected++;
             return TURTLE_ERR_BUDGET;
         }
@@ -283,7 +340,7 @@ int turtle_server_submit(turtle_server_handle sv, int dtype, void *keys, int32_t
     return TURTLE_OK;
 }
 
-int turtle_server_wait(turtle_server_handle sv, uint64_t ticket, double *latency_seconds)
+int turtle_server_wait_info(turtle_server_handle sv, uint64_t ticket, turtle_request_info *info)
 {
     if (!sv) return TURTLE_ERR_INVALID;
     std::unique_lock<std::mutex> lk(sv->mu);
@@ -292,8 +349,23 @@ int turtle_server_wait(turtle_server_handle sv, uint64_t ticket, double *latency
     std::shared_ptr<Request> r = it->second;
     sv->cv_done.wait(lk, [&] { return r->done; });
     sv->live.erase(ticket);
-    if (latency_seconds) *latency_seconds = r->t_done - r->t_submit;
+    if (info) {
+        info->latency_seconds = r->t_done - r->t_submit;
+        info->queued_seconds = r->n ? r->t_start - r->t_submit : 0.0;
+        info->batch_seconds = r->n ? r->t_done - r->t_start : 0.0;
+        info->batch_keys = r->batch_keys;
+        info->batch_requests = r->batch_requests;
+        info->status = r->status;
+    }
     return r->status;
+}
+
+int turtle_server_wait(turtle_server_handle sv, uint64_t ticket, double *latency_seconds)
+{
+    turtle_request_info info{};
+    const int rc = turtle_server_wait_info(sv, ticket, &info);
+    if (latency_seconds && rc != TURTLE_ERR_STATE && rc != TURTLE_ERR_INVALID) *latency_seconds = info.latency_seconds;
+    return rc;
 }
 
 int turtle_server_test(turtle_server_handle sv, uint64_t ticket, int *done)
