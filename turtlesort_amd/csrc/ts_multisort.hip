@@ -225,6 +225,109 @@ __global__ void __launch_bounds__(NT) ms_small(const U *__restrict__ in, U *__re
     }
 }
 
+// ---- arrays of 513..1024 keys of up to 4 bytes: one wave per array, LDS radix ------
+// Each wave sorts one array (four waves per workgroup, no block barrier):
+// its keys are loaded into registers (16 per lane, lane l holding positions
+// i*64 + l), and stable LSD passes of 8-bit digits over the key bits that
+// differ within the array sort them: per pass a per-wave counter per digit
+// gives every key its rank with one LDS atomic (rank16_add: the wave's
+// keys of a digit in (item, lane) order, i.e. in position order), a
+// 64-lane scan of the 256 counters gives each digit's first slot, and the
+// keys are placed in LDS and read back in position order.  Positions past
+// the array's end hold the largest key and stay behind its keys.
+template <typename U, bool FL, bool IDX>
+__global__ void __launch_bounds__(256) ms_radix(const U *__restrict__ in, U *__restrict__ out,
+                                                 const uint32_t *__restrict__ iin, uint32_t *__restrict__ iout,
+                                                 uint32_t arr_size, uint64_t num_arrays, U flip)
+{
+    static_assert(sizeof(U) <= 4, "keys of up to 4 bytes");
+    constexpr int WPB = 4, IT = 16;
+    constexpr uint32_t NS = WAVE * IT;       // 1024 slots per wave
+    __shared__ U sk[WPB][NS];
+    __shared__ uint32_t si[IDX ? WPB : 1][IDX ? NS : 1];
+    __shared__ uint32_t cnt[WPB][128];       // 256 16-bit digit counters per wave
+    const uint32_t w = threadIdx.x / WAVE, l = lane();
+    const uint64_t arr = (uint64_t)blockIdx.x * WPB + w;
+    if (arr >= num_arrays) return;           // wave-uniform; no block barrier below
+    const size_t base = (size_t)arr * arr_size;
+    U k[IT];
+    uint32_t v[IT];
+#pragma unroll
+    for (int i = 0; i < IT; i++) {
+        const uint32_t j = (uint32_t)i * WAVE + l;
+        k[i] = j < arr_size ? in[base + j] : (U)0;
+        v[i] = IDX && iin && j < arr_size ? iin[base + j] : j;
+    }
+    // bits that differ among the array's keys: a pass whose digit is the
+    // same for every key is skipped
+    U band = key_max<U>(), bor = 0;
+#pragma unroll
+    for (int i = 0; i < IT; i++) {
+        const uint32_t j = (uint32_t)i * WAVE + l;
+        k[i] = j < arr_size ? to_ord<U, FL>(k[i], flip) : key_max<U>();
+        if (j < arr_size) {
+            band &= k[i];
+            bor |= k[i];
+        }
+    }
+#pragma unroll
+    for (int d = WAVE / 2; d >= 1; d >>= 1) {
+        band &= (U)__shfl_xor((uint32_t)band, d, WAVE);
+        bor |= (U)__shfl_xor((uint32_t)bor, d, WAVE);
+    }
+    const uint32_t diff = (uint32_t)(band ^ bor);
+    U *const ks = sk[w];
+    uint32_t *const vs = si[IDX ? w : 0];
+    uint32_t *const cw = cnt[w];
+    for (uint32_t sh = 0; sh < 8 * sizeof(U); sh += 8) {
+        if (((diff >> sh) & 0xffu) == 0) continue;   // wave-uniform
+        cw[l] = 0;
+        cw[l + WAVE] = 0;
+        wave_sync();
+        uint32_t dr[IT];
+#pragma unroll
+        for (int i = 0; i < IT; i++) {
+            const uint32_t d = ((uint32_t)k[i] >> sh) & 0xffu;
+            dr[i] = d | (rank16_add(cw, d, 1u) << 8);
+        }
+        wave_sync();
+        // exclusive scan of the 256 counters: lane l owns digits 4l .. 4l+3
+        // (the two halves of words 2l and 2l+1)
+        {
+            const uint32_t w0 = cw[2 * l], w1 = cw[2 * l + 1];
+            const uint32_t c0 = w0 & 0xffffu, c1 = w0 >> 16, c2 = w1 & 0xffffu, c3 = w1 >> 16;
+            const uint32_t tot = c0 + c1 + c2 + c3;
+            const uint32_t p0 = wave_incl_scan(tot) - tot, p1 = p0 + c0, p2 = p1 + c1, p3 = p2 + c2;
+            cw[2 * l] = p0 | (p1 << 16);
+            cw[2 * l + 1] = p2 | (p3 << 16);
+        }
+        wave_sync();
+#pragma unroll
+        for (int i = 0; i < IT; i++) {
+            const uint32_t d = dr[i] & 0xffu;
+            const uint32_t first = (cw[d >> 1] >> ((d & 1u) << 4)) & 0xffffu;
+            const uint32_t pos = (first + (dr[i] >> 8)) & (NS - 1);
+            ks[pos] = k[i];
+            if (IDX) vs[pos] = v[i];
+        }
+        wave_sync();
+#pragma unroll
+        for (int i = 0; i < IT; i++) {
+            k[i] = ks[(uint32_t)i * WAVE + l];
+            if (IDX) v[i] = vs[(uint32_t)i * WAVE + l];
+        }
+        wave_sync();
+    }
+#pragma unroll
+    for (int i = 0; i < IT; i++) {
+        const uint32_t j = (uint32_t)i * WAVE + l;
+        if (j < arr_size) {
+            out[base + j] = from_ord<U, FL>(k[i], flip);
+            if (IDX) iout[base + j] = v[i];
+        }
+    }
+}
+
 template <typename U, bool FL, bool IDX, int N>
 static void small_launch(uint32_t arr_size, const U *in, U *out, const uint32_t *iin, uint32_t *iout, uint64_t na,
                          U f, hipStream_t st)
@@ -262,6 +365,11 @@ static hipError_t ms_launch(uint64_t flip, uint32_t arr_size, const void *in, vo
             case 16: small_launch<U, FL, IDX, 16>(arr_size, ii, oo, xi, xo, na, f, st); break;
             default: small_launch<U, FL, IDX, 32>(arr_size, ii, oo, xi, xo, na, f, st); break;
             }
+        } else if (ne == 1024 && sizeof(U) <= 4) {
+            // 513..1024 keys of up to 4 bytes: LDS radix, one wave per array
+            if constexpr (sizeof(U) <= 4)
+                hipLaunchKernelGGL((ms_radix<U, FL, IDX>), dim3((uint32_t)((na + 3) / 4)), dim3(256), 0, st, ii, oo,
+                                   xi, xo, arr_size, na, f);
         } else if (ne <= 1024) {
             // one wave of 16 keys per lane, 1024 / ne arrays per wave (no
             // cross-wave barriers; 16 keys per thread take one merge round
