@@ -200,6 +200,35 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *red, u
     return wpre + inc - v;
 }
 
+// ---- range-checked loads (buffer instructions) ------------------------------------
+// A raw buffer resource over [base, base + bytes): loads at byte offsets at or
+// past `bytes` return 0 instead of touching memory, so a tile's last partial
+// wave needs no per-item predicate or clamped address, and full and partial
+// tiles share one load sequence (scalar base, one offset register, constant
+// offsets).  Word 3 = 0x00020000: 32-bit data format, no swizzle, no index
+// stride, the raw-buffer setting for gfx9-family parts.
+typedef __amdgpu_buffer_rsrc_t BufRsrc;
+
+__device__ __forceinline__ BufRsrc buf_rsrc(const void *base, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, 0x00020000);
+}
+
+template <typename T>
+__device__ __forceinline__ T buf_load(BufRsrc r, uint32_t off)
+{
+    if constexpr (sizeof(T) == 8) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0);
+        return (T)(((uint64_t)v[1] << 32) | v[0]);
+    } else if constexpr (sizeof(T) == 4) {
+        return (T)__builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0);
+    } else if constexpr (sizeof(T) == 2) {
+        return (T)__builtin_amdgcn_raw_buffer_load_b16(r, (int)off, 0, 0);
+    } else {
+        return (T)__builtin_amdgcn_raw_buffer_load_b8(r, (int)off, 0, 0);
+    }
+}
+
 // ---- LDS layout with one padding slot every ITEMS elements -------------------
 // Thread t of a blocked arrangement touches elements [t*ITEMS, (t+1)*ITEMS);
 // the padding makes the stride ITEMS+1 (odd), so lanes hit distinct banks.

@@ -43,6 +43,7 @@
 // later kernel on the same stream.  Cross-workgroup writes inside a kernel
 // are atomic appends to lists that only later kernels read.
 #pragma once
+#include <type_traits>
 #include "ts_device.h"
 #include "ts_engine.h"
 #include "ts_internal.h"
@@ -76,6 +77,11 @@ struct Cfg {
     static constexpr uint32_t SORT_KEY_BYTES = (padded_len<SI>(CAP) * KB + 15) & ~15u;
     static constexpr uint32_t SORT_BYTES = SORT_KEY_BYTES + (IDX ? padded_len<SI>(CAP) * 4 : 0);
     static constexpr uint32_t SCAT_BYTES = TILE * KB + TILE * IB + TILE;
+    // count pass: bucket table over the segment's key range (S >= 15; for
+    // S <= 7 the splitters are compared in registers)
+    static constexpr bool USE_LUT = S >= 15;
+    static constexpr int LUTB = 10;
+    static constexpr uint32_t NLUT = 1u << LUTB;
     static_assert((S + 1) == (1 << LOG), "S + 1 must be a power of two");
     static_assert(S <= MAX_SPLIT && NBP <= WAVE, "too many splitters");
     static_assert(CLAST < LDS_CLASSES && (1 << CMIN) >= SI && CMIN < CRAD && WS * NW == CAP, "size classes");
@@ -283,6 +289,26 @@ struct LevelK {
         uint32_t begin, len, depth;
     };
 
+    // Bucket table of the count pass.  The order values [lo, hi] of a
+    // segment are cut into NLUT buckets of 2^sh values each; the entry of a
+    // bucket holds lt0 = the number of splitters below the bucket (bits
+    // 0-5), 0, 1 or "2 or more" splitters inside it (bits 6-7) and, with
+    // exactly one inside, that splitter's offset from the bucket's first
+    // value (bits 8 up; all ones otherwise).  A key's bin is then one LDS
+    // load and two compares; buckets with several splitters (and keys
+    // outside [lo, hi]) take the tree search.
+    typedef typename std::conditional<KB == 8, uint64_t, uint32_t>::type LutE;
+    static constexpr uint32_t NLUT = C::USE_LUT ? C::NLUT : 1u;
+    static constexpr LutE LUT_NOOFF = (LutE)(~(LutE)0 >> 8);   // offset field of "no single splitter"
+    static constexpr LutE LUT_MANY = (LutE)0x80u;
+    struct SharedCount {
+        uint32_t wcnt[2][NW][NBP];   // per-wave bin counts (two sets, alternating between tiles)
+        Split<U, S> sp;
+        LutE lut[NLUT];
+        U lut_lo, lut_span;
+        uint32_t lut_sh;
+    };
+
     static __device__ __forceinline__ void load_reg(const Split<U, S> &sp, Reg &reg, U (&reg3)[3])
     {
         if constexpr (S <= 7) {
@@ -317,107 +343,340 @@ struct LevelK {
     }
 
     // ---------------------------------------------------------------- count
-    static __device__ void count(const Params &p, SharedTile &sh)
+    // number of splitters below v: the Eytzinger search, all levels in LDS
+    static __device__ __forceinline__ uint32_t tree_lt(const Split<U, S> &sp, U v)
+    {
+        uint32_t i = 1;
+#pragma unroll
+        for (int l = 0; l < LOG; l++) i = 2 * i + ((sp.tree[i] < v) ? 1u : 0u);
+        return i - (S + 1);
+    }
+
+    // fills the bucket table of segment sg (its sorted splitters and search
+    // tree are already in sh.sp)
+    static __device__ void lut_build(SharedCount &sh, const Seg *sg)
+    {
+        const U kmax = key_max<U>();
+        const U lo = (U)sg->lo;
+        const U hi = sg->hi > (uint64_t)kmax ? kmax : (U)sg->hi;
+        const U span = hi > lo ? (U)(hi - lo) : (U)0;
+        const uint32_t bits = span ? (uint32_t)(64 - __clzll((uint64_t)span)) : 0u;
+        const uint32_t shf = bits > (uint32_t)C::LUTB ? bits - (uint32_t)C::LUTB : 0u;
+        const uint64_t wm = (1ull << shf) - 1ull;   // values per bucket - 1 (shf < 64)
+        if (threadIdx.x == 0) {
+            sh.lut_lo = lo;
+            sh.lut_span = span;
+            sh.lut_sh = shf;
+        }
+        for (uint32_t b = threadIdx.x; b < NLUT; b += NT) {
+            const uint64_t o = (uint64_t)b << shf;   // first value of the bucket, relative to lo
+            LutE e = LUT_MANY | (LUT_NOOFF << 8);   // past hi: no key maps here
+            if (o <= (uint64_t)span) {
+                const U bs = (U)(lo + (U)o);
+                const U be = ((uint64_t)(U)(span - (U)o) < wm) ? hi : (U)(bs + (U)wm);
+                const uint32_t lt0 = tree_lt(sh.sp, bs);
+                const uint32_t le = be == kmax ? (uint32_t)S : tree_lt(sh.sp, (U)(be + 1));
+                const uint32_t m = le - lt0;
+                LutE off = LUT_NOOFF;
+                if (m == 1) off = (LutE)(U)(sh.sp.srt[lt0] - bs);
+                e = (LutE)lt0 | (LutE)((m < 2 ? m : 2u) << 6) | (off << 8);
+            }
+            sh.lut[b] = e;
+        }
+    }
+
+    // bins of N keys through the bucket table; keys in buckets with several
+    // splitters or outside [lo, hi] take the tree search.  Groups of 8 keys:
+    // their table loads are issued together and wait once, and fewer
+    // entries are live at a time.
+    template <int N>
+    static __device__ __forceinline__ void classify_lut_n(const U (&x)[N], uint32_t (&bin)[N], const SharedCount &sh,
+                                                          U lo, U span, uint32_t shf, const Reg &reg, const U (&reg3)[3])
+    {
+        // lo, span and shf are the same for every lane (scalar registers);
+        // shf < 8 * KB, so the shifts stay in the key's width
+        const LutE wm = (LutE)(((LutE)1 << shf) - 1u);
+        constexpr int GR = KB == 8 ? (N < 4 ? N : 4) : (N < 8 ? N : 8);
+        static_assert(N % GR == 0, "groups of keys");
+#pragma unroll
+        for (int g0 = 0; g0 < N; g0 += GR) {
+            LutE e[GR];
+#pragma unroll
+            for (int c = 0; c < GR; c++) {
+                const LutE q = (LutE)(U)(x[g0 + c] - lo) >> shf;
+                e[c] = sh.lut[q < (LutE)NLUT ? (uint32_t)q : NLUT - 1];
+            }
+            uint32_t full = 0;
+#pragma unroll
+            for (int c = 0; c < GR; c++) {
+                const U d = (U)(x[g0 + c] - lo);
+                const uint32_t lt0 = (uint32_t)e[c] & 63u;
+                const LutE off = e[c] >> 8;
+                const LutE low = (LutE)d & wm;
+                bin[g0 + c] = 2 * lt0 + (low > off ? 2u : 0u) + (low == off ? 1u : 0u);
+                full |= ((((uint32_t)e[c] & (uint32_t)LUT_MANY) != 0) || d > span) ? (1u << c) : 0u;
+            }
+            if (ballot(full != 0)) {   // wave-uniform
+#pragma unroll
+                for (int c = 0; c < GR; c++)
+                    if ((full >> c) & 1u) bin[g0 + c] = classify<U, S, LOG>(x[g0 + c], sh.sp, reg, reg3);
+            }
+        }
+    }
+
+    // A tile of the current level: its segment, its index inside the
+    // segment, the segment's keys and the tile's key count.
+    struct TileRef {
+        uint32_t sid, t, tl;       // segment, tile inside it, keys in the tile
+        uint32_t kbeg, depth;      // first key of the tile in its level buffer, segment depth
+        uint32_t tile_base, ntiles;
+    };
+
+    // The three key buffers a level reads from (by segment depth), as
+    // opaque scalar values: selecting between the kernel argument's fields
+    // directly lets the compiler turn the selection into a load at a
+    // computed offset inside the argument block, which moves the whole
+    // block to scratch memory.
+    struct KeyBases {
+        uint64_t b[3];
+        __device__ __forceinline__ explicit KeyBases(const Params &p)
+        {
+            b[0] = (uint64_t)p.kin;
+            b[1] = (uint64_t)p.kt0;
+            b[2] = (uint64_t)p.kt1;
+#pragma unroll
+            for (int i = 0; i < 3; i++) asm volatile("" : "+s"(b[i]));
+        }
+        __device__ __forceinline__ const U *src(uint32_t depth) const
+        {
+            return (const U *)(depth == 0 ? b[0] : ((depth & 1) ? b[1] : b[2]));
+        }
+    };
+
+    // a value every lane holds alike, moved to scalar registers
+    static __device__ __forceinline__ U uniform_val(U v)
+    {
+        // (readfirstlane returns int: each half goes through uint32_t, else
+        // the low half would be sign-extended into the high one)
+        if constexpr (KB == 8)
+            return ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32) |
+                   (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
+        else
+            return (U)__builtin_amdgcn_readfirstlane((uint32_t)v);
+    }
+
+    // TileRef of tile g (false, after a range_fail, when it is inconsistent).
+    // prev: the tile before g when it is known, so that a tile of the same
+    // segment needs no table loads.  Every field is block-uniform and is
+    // kept in scalar registers (readfirstlane), so that the key loads take
+    // a scalar base address.
+    static __device__ __forceinline__ bool tile_ref(const Params &p, uint32_t g, const TileRef *prev, TileRef &r)
+    {
+        uint32_t begin, len;
+        if (prev && prev->t + 1 < prev->ntiles) {
+            r.sid = prev->sid;
+            r.t = prev->t + 1;
+            r.tile_base = prev->tile_base;
+            r.ntiles = prev->ntiles;
+            r.depth = prev->depth;
+            const Seg *sg = &p.segs[r.sid];
+            begin = __builtin_amdgcn_readfirstlane(sg->begin);
+            len = __builtin_amdgcn_readfirstlane(sg->len);
+        } else {
+            uint32_t sid, t;
+            if (!tile_of(p, g, sid, t)) {
+                if (threadIdx.x == 0) range_fail(p, 4);
+                return false;
+            }
+            r.sid = __builtin_amdgcn_readfirstlane(sid);
+            r.t = __builtin_amdgcn_readfirstlane(t);
+            const Seg *sg = &p.segs[r.sid];
+            r.tile_base = __builtin_amdgcn_readfirstlane(sg->tile_base);
+            r.ntiles = __builtin_amdgcn_readfirstlane(sg->ntiles);
+            r.depth = __builtin_amdgcn_readfirstlane(sg->depth);
+            begin = __builtin_amdgcn_readfirstlane(sg->begin);
+            len = __builtin_amdgcn_readfirstlane(sg->len);
+        }
+        if (g >= p.bins_tiles) {
+            if (threadIdx.x == 0) range_fail(p, 12);
+            return false;
+        }
+        const uint32_t ts = r.t * TILE;
+        r.tl = (len - ts) < TILE ? (len - ts) : TILE;
+        r.kbeg = begin + ts;
+        return true;
+    }
+
+    // issues the loads of a tile's keys (buffer loads bounded by the tile:
+    // positions past its end read 0, their bins are set to the marker later)
+    static __device__ __forceinline__ void count_load(const KeyBases &kb, const TileRef &r, U (&x)[TI])
+    {
+        // the thread index is re-read through an empty asm statement: the
+        // offsets derived from it are recomputed here instead of being
+        // hoisted out of the tile loop and spilled
+        uint32_t tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        const uint32_t ebase = (tid / WAVE) * (TI * WAVE) + tid % WAVE;
+        const BufRsrc rs = buf_rsrc(kb.src(r.depth) + r.kbeg, r.tl * (uint32_t)KB);
+#pragma unroll
+        for (int c = 0; c < TI; c++) x[c] = buf_load<U>(rs, (ebase + (uint32_t)c * WAVE) * (uint32_t)KB);
+    }
+
+    // Count pass, first half of a tile: the bins of the keys in x (loaded by
+    // count_load), four to a dword in the layout the scatter reads; items
+    // past the tile's end hold the marker NB.
+    static __device__ __forceinline__ void count_classify(const Params &p, SharedCount &sh, const TileRef &r,
+                                                          const Reg &reg, const U (&reg3)[3], U (&x)[TI],
+                                                          uint32_t (&bw)[TI / 4])
+    {
+        const U flip = (U)p.flip;
+        uint32_t bins[TI];
+#pragma unroll
+        for (int c = 0; c < TI; c++) x[c] = to_ord<U, FL>(x[c], flip);
+        if constexpr (C::USE_LUT)
+            classify_lut_n<TI>(x, bins, sh, uniform_val(sh.lut_lo), uniform_val(sh.lut_span),
+                               __builtin_amdgcn_readfirstlane(sh.lut_sh), reg, reg3);
+        else
+            classify_n<U, S, LOG, TI>(x, bins, sh.sp, reg, reg3);
+        if (r.tl < TILE) {   // block-uniform
+            uint32_t tid = threadIdx.x;
+            asm volatile("" : "+v"(tid));
+            const uint32_t ebase = (tid / WAVE) * (TI * WAVE) + tid % WAVE;
+#pragma unroll
+            for (int c = 0; c < TI; c++)
+                if (ebase + (uint32_t)c * WAVE >= r.tl) bins[c] = NB;
+        }
+#pragma unroll
+        for (int q = 0; q < TI / 4; q++)
+            bw[q] = bins[4 * q] | (bins[4 * q + 1] << 8) | (bins[4 * q + 2] << 16) | (bins[4 * q + 3] << 24);
+    }
+
+    // Count pass, second half of a tile: per-wave counters (wc: this tile's
+    // set; two sets alternate between tiles, so one barrier per tile
+    // suffices), the bins stored for the scatter, the tile's counts.
+    // FULL: the tile holds TILE keys.
+    template <bool FULL>
+    static __device__ __forceinline__ void count_accum(const Params &p, uint32_t (*wc)[NBP], const TileRef &r,
+                                                       uint32_t g, const uint32_t (&bw)[TI / 4])
+    {
+        uint32_t tid = threadIdx.x;   // opaque copy: see count_load
+        asm volatile("" : "+v"(tid));
+        const int w = __builtin_amdgcn_readfirstlane(tid / WAVE);
+        const uint32_t ln = tid % WAVE;
+        uint32_t *bout = (uint32_t *)(p.bins + (size_t)g * TILE);
+        const uint32_t tl = FULL ? TILE : r.tl;
+        const uint32_t ebase = (uint32_t)w * (TI * WAVE) + ln;
+        auto binof = [&](int c) { return (bw[c / 4] >> (8 * (c % 4))) & 0xffu; };
+        // An item whose valid lanes all hold one bin (sorted runs, heavy
+        // duplicates) adds its count with one atomic: 64 same-address
+        // atomics would be serialised by the LDS.  The items are tested only
+        // when the wave's first or last item is such a one, so that random
+        // keys pay two tests per tile.
+        auto uniform = [&](int c, uint32_t &b0, uint64_t &vm) {
+            const bool ok = FULL || ebase + (uint32_t)c * WAVE < tl;
+            const uint32_t b = binof(c);
+            b0 = __builtin_amdgcn_readfirstlane(b);
+            vm = ballot(ok);
+            return ballot(ok && b == b0) == vm;
+        };
+        uint32_t bf, bl;
+        uint64_t mf, ml;
+        if (uniform(0, bf, mf) || uniform(TI - 1, bl, ml)) {   // wave-uniform
+#pragma unroll
+            for (int c = 0; c < TI; c++) {
+                uint32_t b0 = 0;
+                uint64_t vm = 0;
+                if (uniform(c, b0, vm)) {
+                    if (ln == 0 && vm) atomicAdd(&wc[w][b0], (uint32_t)__popcll(vm));
+                } else if (FULL || ebase + (uint32_t)c * WAVE < tl) {
+                    atomicAdd(&wc[w][binof(c)], 1u);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < TI; c++)
+                if (FULL || ebase + (uint32_t)c * WAVE < tl) atomicAdd(&wc[w][binof(c)], 1u);
+        }
+        // the scatter reads the bins back
+#pragma unroll
+        for (int q = 0; q < TI / 4; q++) bout[bin_dword(w, q, ln) / 4] = bw[q];
+        block_sync();
+        if (threadIdx.x < (uint32_t)NB) {
+            uint32_t s = 0;
+#pragma unroll
+            for (int i = 0; i < NW; i++) s += wc[i][threadIdx.x];
+            p.tcnt[(size_t)r.tile_base * NB + (size_t)threadIdx.x * r.ntiles + r.t] = s;
+        }
+    }
+
+    // Each workgroup takes a contiguous range of the level's tiles, so that
+    // it changes segment (splitter and bucket table loads, two barriers)
+    // rarely, and issues the loads of every tile's keys as soon as the tile
+    // before is classified, so that they are in flight while that tile's
+    // counters are added up.
+    static __device__ void count(const Params &p, SharedCount &sh)
     {
         const uint32_t ntot = p.ctrl->tile_alloc < p.tilecap ? p.ctrl->tile_alloc : p.tilecap;
-        const uint32_t G = p.tiles_per_wg;
-        const U flip = (U)p.flip;
+        // (the division is done in vector registers: readfirstlane keeps the
+        // range, and so the tile loop, in scalar registers)
+        const uint32_t per = __builtin_amdgcn_readfirstlane(ntot / gridDim.x);
+        const uint32_t rem = ntot - per * gridDim.x;
+        const uint32_t g0 = blockIdx.x * per + (blockIdx.x < rem ? blockIdx.x : rem);
+        const uint32_t g1 = g0 + per + (blockIdx.x < rem ? 1u : 0u);
         const int w = threadIdx.x / WAVE;
-        uint64_t nbytes = 0;
+        const KeyBases kb(p);
+        uint64_t nbytes = 0, ntiles = 0;
         uint32_t cur = ~0u;
         Reg reg;
         U reg3[3];
-        for (uint32_t c0 = blockIdx.x * G; c0 < ntot; c0 += gridDim.x * G) {
-            const uint32_t c1 = c0 + G < ntot ? c0 + G : ntot;
-            for (uint32_t g = c0; g < c1; g++) {
-                uint32_t sid, t;
-                if (!tile_of(p, g, sid, t)) {
-                    if (threadIdx.x == 0) range_fail(p, 4);
-                    continue;
-                }
-                const Seg *sg = &p.segs[sid];
-                if (sid != cur) {
-                    split_load<U, S, LOG>(sh.sp, sg);
+        U x[TI];
+        TileRef r{}, rn{};
+        bool ok = false;
+        uint32_t g = g0;
+        // the first consistent tile of the range
+        while (g < g1 && !(ok = tile_ref(p, g, nullptr, r))) g++;
+        if (ok) count_load(kb, r, x);
+        uint32_t par = 0;
+        while (g < g1) {
+            if (r.sid != cur) {
+                split_load<U, S, LOG>(sh.sp, &p.segs[r.sid]);
+                block_sync();
+                if constexpr (C::USE_LUT) {
+                    lut_build(sh, &p.segs[r.sid]);
                     block_sync();
-                    cur = sid;
                 }
-                load_reg(sh.sp, reg, reg3);
-                const U *src = src_keys<U>(p, sg->depth) + sg->begin;
-                if (g >= p.bins_tiles) {
-                    if (threadIdx.x == 0) range_fail(p, 12);
-                    continue;
-                }
-                uint32_t *bout = (uint32_t *)(p.bins + (size_t)g * TILE);
-                const uint32_t ts = t * TILE;
-                const uint32_t tl = (sg->len - ts) < TILE ? (sg->len - ts) : TILE;
-                nbytes += (uint64_t)tl * (KB + 1);
-                if (lane() < (uint32_t)NBP) sh.wcnt[w][lane()] = 0;
-                wave_sync();
-                U x[TI];
-#pragma unroll
-                for (int c = 0; c < TI; c++) {
-                    const uint32_t e = (uint32_t)w * (TI * WAVE) + c * WAVE + lane();
-                    x[c] = e < tl ? src[ts + e] : (U)0;
-                }
-                // all searches first (loads only, so they overlap), then the
-                // counter updates: one LDS atomic per key, conflicts between
-                // lanes of equal bins are serialised by the LDS
-                uint32_t bins[TI];
-#pragma unroll
-                for (int c = 0; c < TI; c++) x[c] = to_ord<U, FL>(x[c], flip);
-                classify_n<U, S, LOG, TI>(x, bins, sh.sp, reg, reg3);
-                // An item whose valid lanes all hold one bin (sorted runs,
-                // heavy duplicates) adds its count with one atomic: 64
-                // same-address atomics would be serialised by the LDS.  The
-                // items are tested only when the wave's first or last item
-                // is such a one, so that random keys pay two tests per tile.
-                const uint32_t ebase = (uint32_t)w * (TI * WAVE) + lane();
-#pragma unroll
-                for (int c = 0; c < TI; c++)
-                    if (ebase + (uint32_t)c * WAVE >= tl) bins[c] = NB;
-                auto uniform = [&](int c, uint32_t &b0, uint64_t &vm) {
-                    const bool ok = ebase + (uint32_t)c * WAVE < tl;
-                    b0 = __builtin_amdgcn_readfirstlane(bins[c]);
-                    vm = ballot(ok);
-                    return ballot(ok && bins[c] == b0) == vm;
-                };
-                uint32_t bf, bl;
-                uint64_t mf, ml;
-                if (uniform(0, bf, mf) || uniform(TI - 1, bl, ml)) {   // wave-uniform
-#pragma unroll
-                    for (int c = 0; c < TI; c++) {
-                        uint32_t b0 = 0;
-                        uint64_t vm = 0;
-                        if (uniform(c, b0, vm)) {
-                            if (lane() == 0 && vm) atomicAdd(&sh.wcnt[w][b0], (uint32_t)__popcll(vm));
-                        } else if (ebase + (uint32_t)c * WAVE < tl) {
-                            atomicAdd(&sh.wcnt[w][bins[c]], 1u);
-                        }
-                    }
-                } else {
-#pragma unroll
-                    for (int c = 0; c < TI; c++)
-                        if (ebase + (uint32_t)c * WAVE < tl) atomicAdd(&sh.wcnt[w][bins[c]], 1u);
-                }
-                // the scatter reads the bins back (items past the tile's end
-                // hold the marker NB)
-#pragma unroll
-                for (int q = 0; q < TI / 4; q++)
-                    bout[bin_dword(w, q, lane()) / 4] = bins[4 * q] | (bins[4 * q + 1] << 8) | (bins[4 * q + 2] << 16) |
-                                                        (bins[4 * q + 3] << 24);
-                block_sync();
-                if (threadIdx.x < (uint32_t)NB) {
-                    uint32_t s = 0;
-#pragma unroll
-                    for (int i = 0; i < NW; i++) s += sh.wcnt[i][threadIdx.x];
-                    p.tcnt[(size_t)sg->tile_base * NB + (size_t)threadIdx.x * sg->ntiles + t] = s;
-                }
-                block_sync();
+                cur = r.sid;
             }
+            load_reg(sh.sp, reg, reg3);
+            uint32_t (*wc)[NBP] = sh.wcnt[par];
+            par ^= 1u;
+            if (lane() < (uint32_t)NBP) wc[w][lane()] = 0;
+            wave_sync();
+            nbytes += (uint64_t)r.tl * (KB + 1);
+            ntiles++;
+            uint32_t bw[TI / 4];
+            count_classify(p, sh, r, reg, reg3, x, bw);
+            // the next consistent tile of the range, its loads issued (the
+            // scheduling barrier keeps them after the classification: both
+            // tiles' keys would otherwise be live at once)
+            __builtin_amdgcn_sched_barrier(0);
+            uint32_t gn = g + 1;
+            bool okn = false;
+            const TileRef *pv = &r;
+            while (gn < g1 && !(okn = tile_ref(p, gn, pv, rn))) {
+                gn++;
+                pv = nullptr;
+            }
+            if (okn) count_load(kb, rn, x);
+            if (r.tl == TILE) count_accum<true>(p, wc, r, g, bw);
+            else count_accum<false>(p, wc, r, g, bw);
+            g = gn;
+            r = rn;
         }
+        block_sync();
         if (threadIdx.x == 0 && nbytes) {
             atomic_add64(&p.ctrl->bytes_count, nbytes);
-            atomic_add64(&p.ctrl->tasks[T_COUNT], 1ull);
+            atomic_add64(&p.ctrl->tasks[T_COUNT], ntiles ? 1ull : 0ull);
         }
     }
 
@@ -1462,7 +1721,7 @@ template <int KT, bool IDX, int S>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kt_bytes(KT) == 1 ? 4 : 6))) k_count(Params p)
 {
     typedef LevelK<KT, IDX, S> K;
-    __shared__ typename K::SharedTile sh;
+    __shared__ typename K::SharedCount sh;
     K::count(p, sh);
 }
 
