@@ -158,7 +158,7 @@ struct EngineTables {
     uint32_t *tcnt = nullptr, *tpre = nullptr, *spart = nullptr;
     uint32_t *tile_seg = nullptr;
     ts::LdsTask *ldst = nullptr;
-    uint32_t *ldsq = nullptr;
+    ts::LdsTask *ldsq = nullptr;
     ts::Level *levels = nullptr;
     uint8_t *bins = nullptr;    // per-key bin of the current level (count -> scatter), per tile
     uint32_t segcap = 0, tilecap = 0, ldscap = 0;
@@ -292,7 +292,7 @@ int ensure_tables(const turtle_sorter *h, const turtle_tuning &tu, EngineTables 
     HIPCHK(hipMalloc((void **)&t.spart, (entries / ts::SCAN_CHUNK + 2) * sizeof(uint32_t)), "hipMalloc(scan sums)");
     HIPCHK(hipMalloc((void **)&t.tile_seg, tilecap * sizeof(uint32_t)), "hipMalloc(tile map)");
     HIPCHK(hipMalloc((void **)&t.ldst, ldscap * sizeof(ts::LdsTask)), "hipMalloc(lds tasks)");
-    HIPCHK(hipMalloc((void **)&t.ldsq, ldscap * sizeof(uint32_t)), "hipMalloc(lds order)");
+    HIPCHK(hipMalloc((void **)&t.ldsq, ldscap * sizeof(ts::LdsTask)), "hipMalloc(lds order)");
     HIPCHK(hipMalloc((void **)&t.levels, ts::MAX_LEVELS * sizeof(ts::Level)), "hipMalloc(levels)");
     HIPCHK(hipMalloc((void **)&t.bins, bins_tiles * tile), "hipMalloc(bins)");
     t.bins_tiles = (uint32_t)bins_tiles;
@@ -1484,7 +1484,7 @@ constexpr uint64_t NA_O_SPART = NA_O_TPRE + na_al(NA_TILECAP * ts::MAX_BINS * 4)
 constexpr uint64_t NA_O_TSEG = NA_O_SPART + na_al((NA_TILECAP * ts::MAX_BINS / ts::SCAN_CHUNK + 2) * 4);
 constexpr uint64_t NA_O_LDST = NA_O_TSEG + na_al(NA_TILECAP * 4);
 constexpr uint64_t NA_O_LDSQ = NA_O_LDST + na_al(NA_LDSCAP * sizeof(ts::LdsTask));
-constexpr uint64_t NA_O_LEVELS = NA_O_LDSQ + na_al(NA_LDSCAP * 4);
+constexpr uint64_t NA_O_LEVELS = NA_O_LDSQ + na_al(NA_LDSCAP * sizeof(ts::LdsTask));
 constexpr uint64_t NA_O_BINS = NA_O_LEVELS + na_al(ts::MAX_LEVELS * sizeof(ts::Level));
 constexpr uint64_t NA_BYTES = NA_O_BINS + na_al(NA_BINS);
 static_assert(NA_BYTES < (64ull << 20), "the allocation-free scratch stays below 64 MiB");
@@ -1576,7 +1576,7 @@ int turtle_sort_device_noalloc(int dtype, const void *keys_in, const int32_t *id
     p.spart = (uint32_t *)(b + NA_O_SPART);
     p.tile_seg = (uint32_t *)(b + NA_O_TSEG);
     p.ldst = (ts::LdsTask *)(b + NA_O_LDST);
-    p.ldsq = (uint32_t *)(b + NA_O_LDSQ);
+    p.ldsq = (ts::LdsTask *)(b + NA_O_LDSQ);
     p.levels = (ts::Level *)(b + NA_O_LEVELS);
     p.bins = b + NA_O_BINS;
     p.segcap = (uint32_t)NA_SEGCAP;

@@ -229,6 +229,24 @@ __device__ __forceinline__ T buf_load(BufRsrc r, uint32_t off)
     }
 }
 
+// store at a byte offset of a buffer resource; stores at or past its end are
+// dropped
+template <typename T>
+__device__ __forceinline__ void buf_store(BufRsrc r, uint32_t off, T v)
+{
+    if constexpr (sizeof(T) == 8) {
+        typedef uint32_t v2u __attribute__((ext_vector_type(2)));
+        const v2u d = {(uint32_t)(uint64_t)v, (uint32_t)((uint64_t)v >> 32)};
+        __builtin_amdgcn_raw_buffer_store_b64(d, r, (int)off, 0, 0);
+    } else if constexpr (sizeof(T) == 4) {
+        __builtin_amdgcn_raw_buffer_store_b32((uint32_t)v, r, (int)off, 0, 0);
+    } else if constexpr (sizeof(T) == 2) {
+        __builtin_amdgcn_raw_buffer_store_b16((uint16_t)v, r, (int)off, 0, 0);
+    } else {
+        __builtin_amdgcn_raw_buffer_store_b8((uint8_t)v, r, (int)off, 0, 0);
+    }
+}
+
 // ---- LDS layout with one padding slot every ITEMS elements -------------------
 // Thread t of a blocked arrangement touches elements [t*ITEMS, (t+1)*ITEMS);
 // the padding makes the stride ITEMS+1 (odd), so lanes hit distinct banks.

@@ -106,7 +106,7 @@ struct Params {
     uint32_t *spart;          // per-chunk sums of the level scan
     uint32_t *tile_seg;       // segment id of every tile of the current level
     LdsTask *ldst;
-    uint32_t *ldsq;           // LDS tasks ordered by size class
+    LdsTask *ldsq;            // the LDS tasks' records ordered by size class
     uint8_t *bins;            // bin of every key of the current level (count -> scatter),
                               // TILE bytes per tile of the level (see bin_slot)
     Level *levels;

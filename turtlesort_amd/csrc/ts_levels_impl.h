@@ -1077,7 +1077,7 @@ struct LevelK {
         for (uint32_t k = a + threadIdx.x; k < b; k += blockDim.x) {
             const uint32_t c = lds_class(p.ldst[k].len);
             const uint32_t slot = coff[c] + atomicAdd(&wbase[c], 1u);   // LDS: a distinct slot per task
-            if (slot < p.ldscap) p.ldsq[slot] = k;
+            if (slot < p.ldscap) p.ldsq[slot] = p.ldst[k];
             else range_fail(p, 11);
         }
     }
@@ -1122,14 +1122,9 @@ struct LevelK {
         uint32_t coff[LDS_CLASSES];       // LDS-sort plan (lds_plan)
         uint32_t cwg[LDS_CLASSES + 1];
         uint32_t mix;                     // mixed 3+3+2-wave steps (lds_plan)
-        // the step's tasks: task of wave w (NW when idle), first wave and
-        // waves of task t (tasks hold consecutive waves; tw0[nk] = waves used)
-        uint32_t wq[NW], tw0[NW + 1], tkw[NW];
         alignas(4) uint16_t cnt[NW][NBIN];   // per-wave digit counts (rank16_add), then the digits' first slots
-        U wmin[NW], wmax[NW];
-        U tmin[NW];
+        U wmin[NW], wmax[NW];             // per-wave key extremes of the step
         uint32_t red[16];
-        uint32_t maxbits;
     };
 
     struct PivotShared {
@@ -1167,56 +1162,93 @@ struct LevelK {
         return x;
     }
 
-    // One workgroup step of the radix classes: nk tasks, task t holding the
-    // rs.tkw[t] consecutive waves from rs.tw0[t] (one class, or the mixed
-    // 3+3+2 steps); wave w holds items i at task position
-    // (w - tw0)*WS + i*WAVE + lane, read from global memory straight into
-    // registers.  The step's LDS arrays are in slot order (task t at slots
-    // [tw0[t]*WS, tw0[t+1]*WS)); on return they hold every task's keys
-    // sorted at the start of its slots.
-    static __device__ void radix_step(const Params &p, U *sk, uint32_t *si, const TaskRec *rec, uint32_t nk,
-                                      uint32_t tid, RadixShared &rs)
+    // The radix classes' workgroup steps.  Step v of class c (kw = c - CRAD
+    // + 1 waves per task, P = NW / kw tasks) holds the class's tasks
+    // first..first+nkc-1 at waves t*kw .. t*kw+kw-1; a mixed step (the first
+    // `mix` steps of the 3-wave class) also holds one 2-wave task at waves
+    // nkc*kw, nkc*kw+1.  Every wave derives its own task from these numbers
+    // and reads the task's record itself (the class-ordered list holds the
+    // records), so no workgroup-wide task table is built.
+    struct RadixWave {
+        LdsTask tk;        // the wave's task (len 0 when the wave is idle)
+        uint32_t wb, kq;   // the task's first wave and its waves
+        uint32_t used;     // waves of the step in use (tasks hold waves [0, used))
+        uint32_t nkc, kw;  // tasks of the class in the step, waves per such task
+        bool mixed, act;
+    };
+
+    static __device__ __forceinline__ RadixWave radix_wave(const Params &p, const RadixShared &rs, uint32_t c,
+                                                           uint32_t sv, uint32_t w)
+    {
+        RadixWave rw;
+        rw.kw = c - CRAD + 1;
+        const uint32_t P = (uint32_t)NW / rw.kw;
+        rw.mixed = rw.kw == 3 && sv < rs.mix;
+        const uint32_t first = sv * P + (rw.kw == 2 ? rs.mix : 0u);
+        const uint32_t cnt = p.ctrl->cls_cnt[c];
+        rw.nkc = first < cnt ? ((cnt - first) < P ? (cnt - first) : P) : 0u;
+        rw.used = rw.nkc * rw.kw + (rw.mixed ? 2u : 0u);
+        uint32_t slot = ~0u;
+        rw.act = w < rw.used;
+        if (w < rw.nkc * rw.kw) {
+            const uint32_t q = w / rw.kw;
+            rw.wb = q * rw.kw;
+            rw.kq = rw.kw;
+            slot = rs.coff[c] + first + q;
+        } else {
+            rw.wb = rw.nkc * rw.kw;
+            rw.kq = rw.act ? 2u : 1u;
+            if (rw.act) slot = rs.coff[c - 1] + sv;   // the mixed step's 2-wave task
+        }
+        rw.tk = LdsTask{0, 0, 0, 0};
+        if (rw.act) {
+            if (slot < p.ldscap) rw.tk = p.ldsq[slot];
+            rw.tk.begin = __builtin_amdgcn_readfirstlane(rw.tk.begin);
+            rw.tk.len = __builtin_amdgcn_readfirstlane(rw.tk.len);
+            rw.tk.depth = __builtin_amdgcn_readfirstlane(rw.tk.depth);
+            if (!span_ok(p, rw.tk.begin, rw.tk.len) || rw.tk.len > rw.kq * WS) {
+                if (lane() == 0) range_fail(p, 1);
+                rw.tk.len = 0;
+            }
+        }
+        return rw;
+    }
+
+    // One workgroup step of the radix classes.  Wave w holds items i at task
+    // position j = (w - wb)*WS + i*WAVE + lane, loaded from global memory
+    // straight into registers (range-checked buffer loads: positions past
+    // the task's end read 0 and become padding with the largest digit); the
+    // step's LDS arrays are in slot order (task at slots [wb*WS, (wb+kq)*WS)),
+    // and every pass leaves each task's keys sorted at the start of its
+    // slots.  The wave then writes its own slots' share of the sorted task
+    // (positions j of its slots) with range-checked buffer stores.
+    static __device__ void radix_step(const Params &p, U *sk, uint32_t *si, const RadixWave &rw, uint32_t tid,
+                                      RadixShared &rs)
     {
         const U flip = (U)p.flip;
         static_assert(NW <= 8, "one workgroup step holds at most 8 waves");
         const uint32_t w = __builtin_amdgcn_readfirstlane(tid / WAVE), l = tid % WAVE;
-        const uint32_t q = __builtin_amdgcn_readfirstlane(rs.wq[w]);   // task of this wave
-        const bool act = q < nk;                          // waves past the step's tasks idle
-        const uint32_t wb = act ? __builtin_amdgcn_readfirstlane(rs.tw0[q]) : 0u;   // task's first wave
+        const bool act = rw.act;
+        const uint32_t wb = rw.wb, len = rw.tk.len;
         const uint32_t j0 = (w - wb) * WS + l;            // task position of item i: j0 + i*WAVE
-        const TaskRec tr = act ? rec[q] : TaskRec{0, 0, 0};
         U *const rk = sk + w * WS + l;                    // slot of item i: rk[i*WAVE]
         uint32_t *const ri = si + w * WS + l;
         U k[SI];
         uint32_t v[SI];
         U mn = key_max<U>(), mx = 0;
-        if (act) {   // wave-uniform
-            // every load is issued first (positions past the task's end
-            // re-read its last key), then the predicates are applied, so the
-            // loads' latencies overlap
-            const U *src = src_keys<U>(p, tr.depth) + tr.begin;
-            const uint32_t *isrc = src_idx(p, tr.depth) + tr.begin;
-            const uint32_t last = tr.len ? tr.len - 1u : 0u;
+        {
+            const BufRsrc kr = buf_rsrc(src_keys<U>(p, rw.tk.depth) + rw.tk.begin, len * (uint32_t)KB);
+            const BufRsrc ir = buf_rsrc(src_idx(p, rw.tk.depth) + rw.tk.begin, IDX ? len * 4u : 0u);
+#pragma unroll
+            for (int i = 0; i < SI; i++) k[i] = buf_load<U>(kr, (j0 + (uint32_t)i * WAVE) * (uint32_t)KB);
+#pragma unroll
+            for (int i = 0; i < SI; i++) v[i] = IDX ? buf_load<uint32_t>(ir, (j0 + (uint32_t)i * WAVE) * 4u) : 0u;
 #pragma unroll
             for (int i = 0; i < SI; i++) {
-                const uint32_t j = j0 + (uint32_t)i * WAVE;
-                const uint32_t jj = j < tr.len ? j : last;
-                k[i] = src[jj];
-                v[i] = IDX ? isrc[jj] : 0u;
-            }
-#pragma unroll
-            for (int i = 0; i < SI; i++) {
-                const bool real = j0 + (uint32_t)i * WAVE < tr.len;
+                const bool real = j0 + (uint32_t)i * WAVE < len;
                 k[i] = real ? to_ord<U, FL>(k[i], flip) : key_max<U>();
-                v[i] = real ? v[i] : 0u;
                 mn = real && k[i] < mn ? k[i] : mn;
                 mx = real && k[i] > mx ? k[i] : mx;
-            }
-        } else {
-#pragma unroll
-            for (int i = 0; i < SI; i++) {
-                k[i] = key_max<U>();
-                v[i] = 0u;
             }
         }
         mn = wave_min(mn);
@@ -1225,22 +1257,31 @@ struct LevelK {
             rs.wmin[w] = mn;
             rs.wmax[w] = mx;
         }
-        if (tid == 0) rs.maxbits = 0;
         block_sync();
-        if (tid < nk) {
-            U a = key_max<U>(), b = 0;
-            for (uint32_t x = rs.tw0[tid]; x < rs.tw0[tid + 1]; x++) {
-                a = rs.wmin[x] < a ? rs.wmin[x] : a;
-                b = rs.wmax[x] > b ? rs.wmax[x] : b;
+        // every wave derives the step's pass count (the largest key span of
+        // its tasks) and its own task's minimum from the per-wave extremes
+        uint32_t maxbits = 0;
+        U tmin = 0;
+        {
+            const uint32_t ntask = rw.nkc + (rw.mixed ? 1u : 0u);
+            for (uint32_t t = 0; t < ntask; t++) {   // uniform
+                const uint32_t a0 = t * rw.kw, a1 = t < rw.nkc ? a0 + rw.kw : a0 + 2u;
+                U a = key_max<U>(), b = 0;
+                for (uint32_t x = a0; x < a1; x++) {
+                    const U xa = rs.wmin[x], xb = rs.wmax[x];
+                    a = xa < a ? xa : a;
+                    b = xb > b ? xb : b;
+                }
+                const uint64_t span = b > a ? (uint64_t)(U)(b - a) : 0ull;
+                const uint32_t bits = span ? (uint32_t)(64 - __clzll(span)) : 0u;
+                maxbits = bits > maxbits ? bits : maxbits;
+                if (a0 == wb) tmin = a;
             }
-            rs.tmin[tid] = a;
-            const uint64_t span = b > a ? (uint64_t)(U)(b - a) : 0ull;
-            atomicMax(&rs.maxbits, span ? (uint32_t)(64 - __clzll(span)) : 0u);
+            maxbits = __builtin_amdgcn_readfirstlane(maxbits);
+            tmin = uniform_val(tmin);
         }
-        block_sync();
-        uint32_t npass = (rs.maxbits + RW - 1) / RW;
+        uint32_t npass = (maxbits + RW - 1) / RW;
         npass = npass < MAXPASS ? npass : MAXPASS;
-        const U tmin = act ? rs.tmin[q] : (U)0;
         // scan ownership: the step's entries in (task, digit, wave of the
         // task) order are NBIN per wave in use, so thread tid owns the EPT
         // consecutive entries from tid*EPT, all inside its own wave's task:
@@ -1248,7 +1289,7 @@ struct LevelK {
         // (e1 + k2) / kq, wave wb + (e1 + k2) % kq; kept as nibbles from
         // (d0, w0).  Threads of idle waves own none.
         static_assert(NBIN == (uint32_t)NT && EPT <= 8 && NW <= 8 && NBIN % EPT == 0, "scan ownership");
-        const uint32_t kq = act ? __builtin_amdgcn_readfirstlane(rs.tkw[q]) : 1u;
+        const uint32_t kq = rw.kq;
         const uint32_t e1 = (w - wb) * NBIN + l * EPT;
         const uint32_t d0 = e1 / kq, w0 = e1 - d0 * kq;
         uint32_t ej = 0, ew = 0;
@@ -1279,7 +1320,7 @@ struct LevelK {
                 wave_sync();
                 // after a pass the task's keys are at its first len positions
                 auto digit = [&](int i) {
-                    const bool real = j0 + (uint32_t)i * WAVE < tr.len;
+                    const bool real = j0 + (uint32_t)i * WAVE < len;
                     return real ? (uint32_t)((U)(k[i] - tmin) >> sh) & (NBIN - 1) : NBIN - 1;
                 };
                 // an item whose lanes share one digit (runs of sorted keys)
@@ -1291,18 +1332,26 @@ struct LevelK {
                     return ballot(d == d0) == ~0ull;
                 };
                 const bool runs = same(digit(0)) || same(digit(SI - 1));   // wave-uniform
+                if (runs) {
 #pragma unroll
-                for (int i = 0; i < SI; i++) {
-                    const uint32_t d = digit(i);
-                    uint32_t r;
-                    if (runs && same(d)) {   // wave-uniform
-                        uint32_t o0 = 0;
-                        if (l == 0) o0 = rank16_add((uint32_t *)rs.cnt[w], d, (uint32_t)WAVE);
-                        r = __builtin_amdgcn_readlane(o0, 0) + l;
-                    } else {
-                        r = rank16_add((uint32_t *)rs.cnt[w], d, 1u);
+                    for (int i = 0; i < SI; i++) {
+                        const uint32_t d = digit(i);
+                        uint32_t r;
+                        if (same(d)) {   // wave-uniform
+                            uint32_t o0 = 0;
+                            if (l == 0) o0 = rank16_add((uint32_t *)rs.cnt[w], d, (uint32_t)WAVE);
+                            r = __builtin_amdgcn_readlane(o0, 0) + l;
+                        } else {
+                            r = rank16_add((uint32_t *)rs.cnt[w], d, 1u);
+                        }
+                        dr[i] = d | (r << 16);
                     }
-                    dr[i] = d | (r << 16);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < SI; i++) {
+                        const uint32_t d = digit(i);
+                        dr[i] = d | (rank16_add((uint32_t *)rs.cnt[w], d, 1u) << 16);
+                    }
                 }
             }
             block_sync();
@@ -1338,24 +1387,73 @@ struct LevelK {
             }
             block_sync();
         }
-        if (npass == 0) {
-            // every task's keys are equal: they stay where they are
-            if (act) {
+        // write-out of the wave's slots: slot w*WS + i*WAVE + l holds task
+        // position j0 + i*WAVE (every task's keys sit at the start of its
+        // slots; with no pass, they are still in registers in load order)
+        if (act) {
+            U *dst = (U *)p.kout;
+            const BufRsrc kw = buf_rsrc(dst + rw.tk.begin, len * (uint32_t)KB);
+            const BufRsrc iw = buf_rsrc(p.iout + rw.tk.begin, IDX ? len * 4u : 0u);
+            if (npass > 0) {
 #pragma unroll
                 for (int i = 0; i < SI; i++) {
-                    rk[i * WAVE] = k[i];
-                    if (IDX) ri[i * WAVE] = v[i];
+                    k[i] = rk[i * WAVE];
+                    if (IDX) v[i] = ri[i * WAVE];
                 }
             }
-            block_sync();
+#pragma unroll
+            for (int i = 0; i < SI; i++) {
+                const uint32_t j = j0 + (uint32_t)i * WAVE;
+                buf_store<U>(kw, j * (uint32_t)KB, from_ord<U, FL>(k[i], flip));
+                if (IDX) buf_store<uint32_t>(iw, j * 4u, v[i]);
+            }
+        }
+        // the next step rewrites the LDS arrays
+        block_sync();
+    }
+
+    // all workgroup steps of the radix classes [CRAD, CLAST]
+    static __device__ void radix_all(const Params &p, unsigned char *arena, RadixShared &rs, unsigned long long *acc)
+    {
+        U *sk = (U *)arena;
+        uint32_t *si = (uint32_t *)(arena + C::SORT_KEY_BYTES);
+        if (threadIdx.x == 0) lds_plan(p.ctrl, CPLAN, NW, CAP, rs.coff, rs.cwg, &rs.mix);
+        if (threadIdx.x == 0) acc[0] = acc[1] = 0;
+        block_sync();
+        const uint32_t vbeg = rs.cwg[CRAD], vend = rs.cwg[CLAST + 1];
+        uint64_t nbytes = 0, ntasks = 0;
+        for (uint32_t v = vbeg + blockIdx.x; v < vend; v += gridDim.x) {
+            // the thread index is re-read through an empty asm statement in
+            // every step: otherwise the compiler hoists the loop-invariant
+            // addresses derived from it out of the loop and spills them
+            uint32_t tid = threadIdx.x;
+            asm volatile("" : "+v"(tid));
+            uint32_t c = CRAD;
+            while (c < CLAST && v >= rs.cwg[c + 1]) c++;
+            c = __builtin_amdgcn_readfirstlane(c);
+            const uint32_t w = __builtin_amdgcn_readfirstlane(tid / WAVE);
+            const RadixWave rw = radix_wave(p, rs, c, __builtin_amdgcn_readfirstlane(v - rs.cwg[c]), w);
+            if (rw.act && w == rw.wb && lane() == 0 && rw.tk.len) {
+                nbytes += 2ull * rw.tk.len * (KB + C::IB);
+                ntasks++;
+            }
+            radix_step(p, sk, si, rw, tid, rs);
+        }
+        if (ntasks) {
+            atomicAdd(&acc[0], (unsigned long long)nbytes);
+            atomicAdd(&acc[1], (unsigned long long)ntasks);
+        }
+        block_sync();
+        if (threadIdx.x == 0 && acc[1]) {
+            atomic_add64(&p.ctrl->bytes_lds, acc[0]);
+            atomic_add64(&p.ctrl->tasks[T_LDS], acc[1]);
         }
     }
 
-    // RAD: the workgroup steps of the radix classes [CRAD, CLAST], else those
-    // of the merge classes [CMIN, CRAD); two launches, disjoint tasks
-    template <bool RAD>
-    static __device__ void lds(const Params &p, unsigned char *arena, TaskRec *rec, RadixShared &rs,
-                               unsigned long long *acc)
+    // the workgroup steps of the merge classes [CMIN, CPLAN) (the radix
+    // classes are the other launch, radix_all: disjoint tasks)
+    static __device__ void lds_merge(const Params &p, unsigned char *arena, TaskRec *rec, RadixShared &rs,
+                                     unsigned long long *acc)
     {
         U *sk = (U *)arena;
         uint32_t *si = (uint32_t *)(arena + C::SORT_KEY_BYTES);
@@ -1364,10 +1462,8 @@ struct LevelK {
         const Ctrl *cc = p.ctrl;
         if (threadIdx.x == 0) lds_plan(cc, CPLAN, NW, CAP, rs.coff, rs.cwg, &rs.mix);
         block_sync();
-        // merge classes [CMIN, CPLAN), radix classes [CRAD, CLAST] (none when
-        // MERGE_ONLY: the merge classes then reach CMAX)
-        const uint32_t c0 = RAD ? (MERGE_ONLY ? (uint32_t)CLAST + 1 : (uint32_t)CRAD) : (uint32_t)C::CMIN,
-                       clast = RAD ? (uint32_t)CLAST : CPLAN - 1;
+        // merge classes [CMIN, CPLAN) (they reach CMAX when MERGE_ONLY)
+        const uint32_t c0 = (uint32_t)C::CMIN, clast = CPLAN - 1;
         const uint32_t vbeg = rs.cwg[c0], vend = rs.cwg[clast + 1];
         uint64_t nbytes = 0, ntasks = 0;
         if (threadIdx.x == 0) acc[0] = acc[1] = 0;
@@ -1380,32 +1476,22 @@ struct LevelK {
             asm volatile("" : "+v"(tid));
             uint32_t c = c0;
             while (c < clast && v >= rs.cwg[c + 1]) c++;
-            const uint32_t kw = RAD ? c - CRAD + 1 : 0u;
-            const uint32_t region = RAD ? kw * WS : 1u << c;     // slots per task of the class
-            const uint32_t P = RAD ? (uint32_t)NW / (kw ? kw : 1u) : CAP >> c;
-            const uint32_t sv = v - rs.cwg[c];                   // step inside the class
-            // the first rs.mix tasks of the 2-wave class ride in the mixed
-            // steps (the first rs.mix steps of the 3-wave class), one each
-            const bool mixed = RAD && kw == 3 && sv < rs.mix;
-            const uint32_t first = sv * P + (RAD && kw == 2 ? rs.mix : 0u);
+            const uint32_t region = 1u << c;     // slots per task of the class
+            const uint32_t P = CAP >> c;
+            const uint32_t sv = v - rs.cwg[c];   // step inside the class
+            const uint32_t first = sv * P;
             const uint32_t cnt = cc->cls_cnt[c];
-            const uint32_t nkc = first < cnt ? ((cnt - first) < P ? (cnt - first) : P) : 0u;
-            const uint32_t nk = nkc + (mixed ? 1u : 0u);
-            // every record the step reads is set: the merge path reads all P
-            // (tasks past nk have length 0), the radix path the nk tasks
-            const uint32_t nrec = RAD ? nk : P;
-            for (uint32_t q = threadIdx.x; q < nrec; q += NT) {
+            const uint32_t nk = first < cnt ? ((cnt - first) < P ? (cnt - first) : P) : 0u;
+            // every record the step reads is set (tasks past nk have length 0)
+            for (uint32_t q = threadIdx.x; q < P; q += NT) {
                 TaskRec r{0, 0, 0};
                 if (q < nk) {
-                    const bool part = q == nkc && mixed;             // the mixed step's 2-wave task
-                    const uint32_t slot = part ? rs.coff[c - 1] + sv : rs.coff[c] + first + q;
-                    const uint32_t reg = part ? 2 * WS : region;
-                    const uint32_t k = slot < p.ldscap ? p.ldsq[slot] : ~0u;
-                    if (k < p.ldscap) {
-                        const LdsTask tk = p.ldst[k];
+                    const uint32_t slot = rs.coff[c] + first + q;
+                    if (slot < p.ldscap) {
+                        const LdsTask tk = p.ldsq[slot];
                         r = TaskRec{tk.begin, tk.len, tk.depth};
                     }
-                    if (!span_ok(p, r.begin, r.len) || r.len > reg) {
+                    if (!span_ok(p, r.begin, r.len) || r.len > region) {
                         range_fail(p, 1);
                         r.len = 0;
                     }
@@ -1414,38 +1500,8 @@ struct LevelK {
                 }
                 rec[q] = r;
             }
-            if (RAD && threadIdx.x == 0) {
-                // waves of the step's tasks, in task order
-                uint32_t W = 0;
-                for (uint32_t t = 0; t < nk; t++) {
-                    const uint32_t kt = t < nkc ? kw : 2u;
-                    rs.tkw[t] = kt;
-                    rs.tw0[t] = W;
-                    for (uint32_t x = 0; x < kt && W < (uint32_t)NW; x++) rs.wq[W++] = t;
-                }
-                rs.tw0[nk] = W;
-                for (uint32_t x = W; x < (uint32_t)NW; x++) rs.wq[x] = NW;   // idle
-            }
             block_sync();
-            if constexpr (RAD) {
-                radix_step(p, sk, si, rec, nk, tid, rs);
-                // write-out: task t's sorted keys are at slots [tw0[t]*WS, tw0[t]*WS + len)
-                uint32_t tid2 = threadIdx.x;
-                asm volatile("" : "+v"(tid2));
-#pragma unroll
-                for (int r = 0; r < SI; r++) {
-                    const uint32_t s = tid2 + (uint32_t)r * NT;
-                    const uint32_t t = rs.wq[s / WS];
-                    if (t < nk) {
-                        const uint32_t j = s - rs.tw0[t] * WS;
-                        const TaskRec tr = rec[t];
-                        if (j < tr.len) {
-                            dst[tr.begin + j] = from_ord<U, FL>(sk[s], flip);
-                            if (IDX) p.iout[tr.begin + j] = si[s];
-                        }
-                    }
-                }
-            } else {
+            {
                 const uint32_t n_eff = region;
                 // the global loads of a batch are issued before its LDS
                 // stores, so they are in flight together (batches of 8 bound
@@ -1784,7 +1840,10 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k
     __shared__ typename K::TaskRec rec[Cfg<KT, IDX, S>::CAP >> Cfg<KT, IDX, S>::CMIN];
     __shared__ typename K::RadixShared rs;
     __shared__ unsigned long long acc[2];
-    K::template lds<RAD>(p, arena, rec, rs, acc);
+    if constexpr (RAD)
+        K::radix_all(p, arena, rs, acc);
+    else
+        K::lds_merge(p, arena, rec, rs, acc);
 }
 
 // ---- host side ------------------------------------------------------------------------
