@@ -430,6 +430,7 @@ struct LevelK {
         uint32_t sid, t, tl;       // segment, tile inside it, keys in the tile
         uint32_t kbeg, depth;      // first key of the tile in its level buffer, segment depth
         uint32_t tile_base, ntiles;
+        uint32_t begin, len;       // the segment
     };
 
     // The three key buffers a level reads from (by segment depth), as
@@ -437,21 +438,37 @@ struct LevelK {
     // directly lets the compiler turn the selection into a load at a
     // computed offset inside the argument block, which moves the whole
     // block to scratch memory.
+    // The key and index buffers a level reads (by segment depth).  The
+    // kernels' only (first) argument is the Params block, at offset 0 of
+    // the kernel argument segment: the selected field is read from there
+    // with a scalar load.  (Selecting between the argument's fields in C++
+    // lets the compiler copy the whole block to scratch memory and index
+    // it; a buffer resource from a value it cannot prove uniform turns
+    // every buffer load into a loop over the lanes.)
     struct KeyBases {
-        uint64_t b[3];
-        __device__ __forceinline__ explicit KeyBases(const Params &p)
+        __device__ __forceinline__ explicit KeyBases(const Params &) {}
+        static __device__ __forceinline__ uint64_t arg_u64(uint32_t off)
         {
-            b[0] = (uint64_t)p.kin;
-            b[1] = (uint64_t)p.kt0;
-            b[2] = (uint64_t)p.kt1;
-#pragma unroll
-            for (int i = 0; i < 3; i++) asm volatile("" : "+s"(b[i]));
+            typedef const __attribute__((address_space(4))) uint64_t CU64;
+            return ((CU64 *)__builtin_amdgcn_kernarg_segment_ptr())[off / 8];
         }
         __device__ __forceinline__ const U *src(uint32_t depth) const
         {
-            return (const U *)(depth == 0 ? b[0] : ((depth & 1) ? b[1] : b[2]));
+            return (const U *)arg_u64(depth == 0 ? (uint32_t)offsetof(Params, kin)
+                                                 : ((depth & 1) ? (uint32_t)offsetof(Params, kt0)
+                                                                : (uint32_t)offsetof(Params, kt1)));
+        }
+        __device__ __forceinline__ const uint32_t *isrc(uint32_t depth) const
+        {
+            return (const uint32_t *)arg_u64(depth == 0 ? (uint32_t)offsetof(Params, iin)
+                                                        : ((depth & 1) ? (uint32_t)offsetof(Params, it0)
+                                                                       : (uint32_t)offsetof(Params, it1)));
         }
     };
+    static_assert(offsetof(Params, kin) % 8 == 0 && offsetof(Params, kt0) % 8 == 0 && offsetof(Params, kt1) % 8 == 0 &&
+                      offsetof(Params, iin) % 8 == 0 && offsetof(Params, it0) % 8 == 0 &&
+                      offsetof(Params, it1) % 8 == 0,
+                  "buffer pointers of the argument block are 8-byte aligned");
 
     // a value every lane holds alike, moved to scalar registers
     static __device__ __forceinline__ U uniform_val(U v)
@@ -470,40 +487,42 @@ struct LevelK {
     // segment needs no table loads.  Every field is block-uniform and is
     // kept in scalar registers (readfirstlane), so that the key loads take
     // a scalar base address.
+    // (the tile and segment tables are read through the constant address
+    // space: scalar loads; the count and scatter passes do not write them)
     static __device__ __forceinline__ bool tile_ref(const Params &p, uint32_t g, const TileRef *prev, TileRef &r)
     {
-        uint32_t begin, len;
+        typedef const __attribute__((address_space(4))) Seg CSeg;
+        typedef const __attribute__((address_space(4))) uint32_t CU32;
         if (prev && prev->t + 1 < prev->ntiles) {
-            r.sid = prev->sid;
+            r = *prev;
             r.t = prev->t + 1;
-            r.tile_base = prev->tile_base;
-            r.ntiles = prev->ntiles;
-            r.depth = prev->depth;
-            const Seg *sg = &p.segs[r.sid];
-            begin = __builtin_amdgcn_readfirstlane(sg->begin);
-            len = __builtin_amdgcn_readfirstlane(sg->len);
         } else {
-            uint32_t sid, t;
-            if (!tile_of(p, g, sid, t)) {
+            const uint32_t sid = ((CU32 *)p.tile_seg)[g];
+            if (sid >= p.segcap) {
                 if (threadIdx.x == 0) range_fail(p, 4);
                 return false;
             }
-            r.sid = __builtin_amdgcn_readfirstlane(sid);
-            r.t = __builtin_amdgcn_readfirstlane(t);
-            const Seg *sg = &p.segs[r.sid];
-            r.tile_base = __builtin_amdgcn_readfirstlane(sg->tile_base);
-            r.ntiles = __builtin_amdgcn_readfirstlane(sg->ntiles);
-            r.depth = __builtin_amdgcn_readfirstlane(sg->depth);
-            begin = __builtin_amdgcn_readfirstlane(sg->begin);
-            len = __builtin_amdgcn_readfirstlane(sg->len);
+            CSeg *sg = (CSeg *)p.segs + sid;
+            r.sid = sid;
+            r.tile_base = sg->tile_base;
+            r.ntiles = sg->ntiles;
+            r.depth = sg->depth;
+            r.begin = sg->begin;
+            r.len = sg->len;
+            r.t = g - r.tile_base;
+            if (g < r.tile_base || r.t >= r.ntiles || !span_ok(p, r.begin, r.len) ||
+                (uint64_t)r.t * TILE >= r.len) {
+                if (threadIdx.x == 0) range_fail(p, 4);
+                return false;
+            }
         }
         if (g >= p.bins_tiles) {
             if (threadIdx.x == 0) range_fail(p, 12);
             return false;
         }
         const uint32_t ts = r.t * TILE;
-        r.tl = (len - ts) < TILE ? (len - ts) : TILE;
-        r.kbeg = begin + ts;
+        r.tl = (r.len - ts) < TILE ? (r.len - ts) : TILE;
+        r.kbeg = r.begin + ts;
         return true;
     }
 
@@ -517,7 +536,9 @@ struct LevelK {
         uint32_t tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
         const uint32_t ebase = (tid / WAVE) * (TI * WAVE) + tid % WAVE;
-        const BufRsrc rs = buf_rsrc(kb.src(r.depth) + r.kbeg, r.tl * (uint32_t)KB);
+        const uint32_t depth = __builtin_amdgcn_readfirstlane(r.depth), kbeg = __builtin_amdgcn_readfirstlane(r.kbeg),
+                       tl = __builtin_amdgcn_readfirstlane(r.tl);   // (uniform resource: see KeyBases)
+        const BufRsrc rs = buf_rsrc(kb.src(depth) + kbeg, tl * (uint32_t)KB);
 #pragma unroll
         for (int c = 0; c < TI; c++) x[c] = buf_load<U>(rs, (ebase + (uint32_t)c * WAVE) * (uint32_t)KB);
     }
@@ -616,9 +637,11 @@ struct LevelK {
     // counters are added up.
     static __device__ void count(const Params &p, SharedCount &sh)
     {
-        const uint32_t ntot = p.ctrl->tile_alloc < p.tilecap ? p.ctrl->tile_alloc : p.tilecap;
-        // (the division is done in vector registers: readfirstlane keeps the
-        // range, and so the tile loop, in scalar registers)
+        // (the tile count comes from a vector load and the division is done
+        // in vector registers: readfirstlane keeps the range, and so the tile
+        // loop and every address derived from it, in scalar registers)
+        const uint32_t ntot = __builtin_amdgcn_readfirstlane(p.ctrl->tile_alloc < p.tilecap ? p.ctrl->tile_alloc
+                                                                                            : p.tilecap);
         const uint32_t per = __builtin_amdgcn_readfirstlane(ntot / gridDim.x);
         const uint32_t rem = ntot - per * gridDim.x;
         const uint32_t g0 = blockIdx.x * per + (blockIdx.x < rem ? blockIdx.x : rem);
@@ -1200,18 +1223,28 @@ struct LevelK {
             rw.kq = rw.act ? 2u : 1u;
             if (rw.act) slot = rs.coff[c - 1] + sv;   // the mixed step's 2-wave task
         }
+        // the record is read through the constant address space: a scalar
+        // load (no kernel writes the list while the LDS sorts run), which the
+        // caller issues a step ahead (radix_check waits for it)
+        typedef const __attribute__((address_space(4))) LdsTask CTask;
         rw.tk = LdsTask{0, 0, 0, 0};
-        if (rw.act) {
-            if (slot < p.ldscap) rw.tk = p.ldsq[slot];
-            rw.tk.begin = __builtin_amdgcn_readfirstlane(rw.tk.begin);
-            rw.tk.len = __builtin_amdgcn_readfirstlane(rw.tk.len);
-            rw.tk.depth = __builtin_amdgcn_readfirstlane(rw.tk.depth);
-            if (!span_ok(p, rw.tk.begin, rw.tk.len) || rw.tk.len > rw.kq * WS) {
-                if (lane() == 0) range_fail(p, 1);
-                rw.tk.len = 0;
-            }
+        if (rw.act && slot < p.ldscap) {
+            CTask *t = (CTask *)p.ldsq + slot;
+            rw.tk.begin = t->begin;
+            rw.tk.len = t->len;
+            rw.tk.depth = t->depth;
         }
         return rw;
+    }
+
+    // validates a wave's task record (a failed check is reported and the
+    // task skipped)
+    static __device__ __forceinline__ void radix_check(const Params &p, RadixWave &rw)
+    {
+        if (rw.act && (!span_ok(p, rw.tk.begin, rw.tk.len) || rw.tk.len > rw.kq * WS)) {
+            if (lane() == 0) range_fail(p, 1);
+            rw.tk.len = 0;
+        }
     }
 
     // One workgroup step of the radix classes.  Wave w holds items i at task
@@ -1422,17 +1455,26 @@ struct LevelK {
         block_sync();
         const uint32_t vbeg = rs.cwg[CRAD], vend = rs.cwg[CLAST + 1];
         uint64_t nbytes = 0, ntasks = 0;
+        // the task records of step v + gridDim.x are loaded while step v runs
+        auto wave_of = [&](uint32_t v) {
+            uint32_t c = CRAD;
+            while (c < CLAST && v >= rs.cwg[c + 1]) c++;
+            c = __builtin_amdgcn_readfirstlane(c);
+            const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+            return radix_wave(p, rs, c, __builtin_amdgcn_readfirstlane(v - rs.cwg[c]), w);
+        };
+        RadixWave rn;
+        if (vbeg + blockIdx.x < vend) rn = wave_of(vbeg + blockIdx.x);
         for (uint32_t v = vbeg + blockIdx.x; v < vend; v += gridDim.x) {
             // the thread index is re-read through an empty asm statement in
             // every step: otherwise the compiler hoists the loop-invariant
             // addresses derived from it out of the loop and spills them
             uint32_t tid = threadIdx.x;
             asm volatile("" : "+v"(tid));
-            uint32_t c = CRAD;
-            while (c < CLAST && v >= rs.cwg[c + 1]) c++;
-            c = __builtin_amdgcn_readfirstlane(c);
+            RadixWave rw = rn;
+            if (v + gridDim.x < vend) rn = wave_of(v + gridDim.x);
+            radix_check(p, rw);
             const uint32_t w = __builtin_amdgcn_readfirstlane(tid / WAVE);
-            const RadixWave rw = radix_wave(p, rs, c, __builtin_amdgcn_readfirstlane(v - rs.cwg[c]), w);
             if (rw.act && w == rw.wb && lane() == 0 && rw.tk.len) {
                 nbytes += 2ull * rw.tk.len * (KB + C::IB);
                 ntasks++;
