@@ -281,6 +281,14 @@ int turtle_set_tuning(turtle_handle h, const turtle_tuning *t);
  * tuning is left as it was. */
 int turtle_autotune(turtle_handle h, int64_t n, int with_index, int reps, turtle_tuning *best, double *best_ms);
 
+/* Autotuner for segmented batches (no reference counterpart: the reference
+ * sorts one array per call): the same candidates timed on synthetic keys
+ * laid out in the given segments (offsets[0] = 0 .. offsets[num_segments],
+ * host memory) through turtle_segmented_sort_device.  Its results are the
+ * "seg" lines of the tuning table, which segmented runs prefer. */
+int turtle_autotune_segmented(turtle_handle h, const int64_t *offsets, int64_t num_segments, int with_index, int reps,
+                              turtle_tuning *best, double *best_ms);
+
 /* ---- request loop with continuous batching ------------------------------ */
 /* No reference counterpart: a reference TurtleSort object holds one sort in
  * flight (turtle-sort.cuh:53-61).  A server carves a device memory budget

@@ -185,3 +185,26 @@ def test_radix_mixed_steps(orc, torch_dev, n2, n3):
         assert st["error"] == 0 and st["segments"] == 0
         check_segments(orc, keys, off, d2h(torch, kout), d2h(torch, iout))
     sorter.close()
+
+
+def test_autotune_segments(orc, torch_dev):
+    """the segmented autotuner returns one of its candidates, and a batch
+    sorted with it equals the oracle"""
+    torch = torch_dev
+    lens = wl.loguniform_lengths(29, 200, 16, 30000)
+    off = wl.segment_offsets(lens)
+    n = int(off[-1])
+    sorter = TurtleSort(np.int64, n)
+    t, ms = sorter.autotune_segments(off, with_index=True, reps=1)
+    cap = 4096
+    assert t["splitters"] in (7, 15, 31) and t["lds_cutoff"] in (cap // 4, cap // 2, cap)
+    assert t["tiles_per_task"] in (1, 2) and ms > 0
+    sorter.set_tuning(**t)
+    keys = wl.gen_keys(np.int64, n, 3, 0)
+    kin = h2d(torch, keys)
+    iin = torch.arange(n, dtype=torch.int32, device="cuda")
+    kout, iout = torch.empty_like(kin), torch.empty_like(iin)
+    sorter.segmented_sort_device(kin, kout, off, iin, iout)
+    sorter.synchronize()
+    check_segments(orc, keys, off, d2h(torch, kout), d2h(torch, iout))
+    sorter.close()

@@ -147,6 +147,8 @@ SIGNATURES = {
     "turtle_get_tuning": (_i32, [_vp, ctypes.POINTER(Tuning)]),
     "turtle_set_tuning": (_i32, [_vp, ctypes.POINTER(Tuning)]),
     "turtle_autotune": (_i32, [_vp, _i64, _i32, _i32, ctypes.POINTER(Tuning), ctypes.POINTER(ctypes.c_double)]),
+    "turtle_autotune_segmented": (_i32, [_vp, _vp, _i64, _i32, _i32, ctypes.POINTER(Tuning),
+                                         ctypes.POINTER(ctypes.c_double)]),
     "turtle_server_create": (_i32, [ctypes.POINTER(ServerConfig), ctypes.POINTER(_vp)]),
     "turtle_server_submit": (_i32, [_vp, _i32, _vp, _vp, _i64, ctypes.POINTER(_u64)]),
     "turtle_server_wait": (_i32, [_vp, _u64, ctypes.POINTER(ctypes.c_double)]),

@@ -307,6 +307,19 @@ class TurtleSort:
                                              ctypes.byref(t), ctypes.byref(ms)))
         return t.as_dict(), ms.value
 
+    def autotune_segments(self, offsets, with_index=True, reps=3):
+        """fastest tuning of a segmented batch laid out by `offsets` (host
+        int64, offsets[0] == 0) on synthetic keys in device memory -> (dict, ms)"""
+        off = np.ascontiguousarray(np.asarray(offsets, dtype=np.int64))
+        if off.ndim != 1 or off.size < 2:
+            raise ValueError("offsets needs at least one segment")
+        t = _lib.Tuning()
+        ms = ctypes.c_double(0.0)
+        _lib.check(self._lib.turtle_autotune_segmented(self._h, off.ctypes.data_as(ctypes.c_void_p), int(off.size - 1),
+                                                       int(bool(with_index)), int(reps), ctypes.byref(t),
+                                                       ctypes.byref(ms)))
+        return t.as_dict(), ms.value
+
     def set_tuning(self, **kw):
         t = _lib.Tuning()
         _lib.check(self._lib.turtle_get_tuning(self._h, ctypes.byref(t)))

@@ -370,7 +370,7 @@ hipStream_t stream_of(turtle_sorter *h, void *stream)
     return (hipStream_t)stream;
 }
 
-turtle_tuning tuning_for(const turtle_sorter *h, bool idx, uint64_t total);
+turtle_tuning tuning_for(const turtle_sorter *h, bool idx, uint64_t total, bool seg);
 
 // longest root of a run: the segments' longest (h->host_off holds the
 // offsets of a segmented run), else the whole input
@@ -488,7 +488,7 @@ int run_engine(turtle_sorter *h, const void *kin, const uint32_t *iin, void *kou
     // sort of its longest segment (a capture uses the tuning its creator
     // resolved and sized the tables for)
     const uint64_t longest = longest_root(h, dev_off != nullptr, total);
-    if (!capturing) h->tune = tuning_for(h, idx, longest);
+    if (!capturing) h->tune = tuning_for(h, idx, longest, dev_off != nullptr);
     EngineTables &tab = capturing ? *capture : h->tab;
     if ((rc = ensure_tables(h, h->tune, tab, total, nseg, 1))) return rc;
     if ((rc = ensure_readback(h))) return rc;
@@ -621,13 +621,17 @@ turtle_tuning default_tuning(int kb)
 // ---- tuning table -------------------------------------------------------------
 // Written by the autotuner (turtlesort_amd/autotune.py over turtle_autotune):
 // lines "kb idx log2n splitters lds_cutoff tiles_per_task workgroups_per_cu
-// ms", '#' comments.  Read once from $TURTLE_TUNING_FILE, else from
-// tuning_table.txt next to this library.  A run of `total` keys uses the
-// entry of its key width and index flag whose log2n is nearest to
-// log2(total); without an entry the built-in default applies.
+// ms", '#' comments; lines that start with "seg" hold the tunings of
+// segmented batches (turtle_autotune_segmented), their log2n being that of
+// the batch's longest segment.  Read once from $TURTLE_TUNING_FILE, else
+// from tuning_table.txt next to this library.  A run uses the entry of its
+// key width and index flag whose log2n is nearest to log2 of its longest
+// root (a segmented run the "seg" entries when there are any); without an
+// entry the built-in default applies.
 struct TuneEntry {
     int kb, idx, lg;
     turtle_tuning t;
+    bool seg;
 };
 
 const std::vector<TuneEntry> &tuning_table()
@@ -654,7 +658,10 @@ const std::vector<TuneEntry> &tuning_table()
             if (line[0] == '#') continue;
             TuneEntry e{};
             double ms = 0;
-            if (sscanf(line, "%d %d %d %d %d %d %d %lf", &e.kb, &e.idx, &e.lg, &e.t.splitters, &e.t.lds_cutoff,
+            const char *q = line;
+            e.seg = strncmp(line, "seg ", 4) == 0;
+            if (e.seg) q += 4;
+            if (sscanf(q, "%d %d %d %d %d %d %d %lf", &e.kb, &e.idx, &e.lg, &e.t.splitters, &e.t.lds_cutoff,
                        &e.t.tiles_per_task, &e.t.workgroups_per_cu, &ms) >= 7)
                 tab.push_back(e);
         }
@@ -665,14 +672,19 @@ const std::vector<TuneEntry> &tuning_table()
 
 bool valid_tuning(const turtle_tuning &t, int kb);
 
-// tuning of a run over `total` keys (the handle's own when it was set by hand)
-turtle_tuning tuning_for(const turtle_sorter *h, bool idx, uint64_t total)
+// tuning of a run whose longest root holds `total` keys (the handle's own
+// when it was set by hand); seg: the run sorts a segmented batch
+turtle_tuning tuning_for(const turtle_sorter *h, bool idx, uint64_t total, bool seg)
 {
     if (h->tune_fixed) return h->tune;
     const int lg = total > 1 ? (int)std::lround(std::log2((double)total)) : 0;
+    bool have_seg = false;
+    if (seg)
+        for (const TuneEntry &e : tuning_table())
+            have_seg |= e.seg && e.kb == h->kb && e.idx == (idx ? 1 : 0) && valid_tuning(e.t, h->kb);
     const TuneEntry *best = nullptr;
     for (const TuneEntry &e : tuning_table()) {
-        if (e.kb != h->kb || e.idx != (idx ? 1 : 0) || !valid_tuning(e.t, h->kb)) continue;
+        if (e.kb != h->kb || e.idx != (idx ? 1 : 0) || e.seg != have_seg || !valid_tuning(e.t, h->kb)) continue;
         if (!best || std::abs(e.lg - lg) < std::abs(best->lg - lg)) best = &e;
     }
     return best ? best->t : h->tune_base;
@@ -964,13 +976,15 @@ int turtle_set_tuning(turtle_handle h, const turtle_tuning *t)
     return TURTLE_OK;
 }
 
-int turtle_autotune(turtle_handle h, int64_t n, int with_index, int reps, turtle_tuning *best, double *best_ms)
+// Times every candidate tuning (splitters 7/15/31 x LDS cutoff cap/4,
+// cap/2, cap x tiles per step 1/2) on synthetic keys (uniform over the key
+// type, iota index) in device memory: one untimed run (tables sized for
+// the tuning), then reps timed runs of run(kin, iin, kout, iout).  The
+// handle's own tuning is left as it was.
+extern "C++" template <typename RUN>
+static int autotune_candidates(turtle_handle h, int64_t n, bool idx, int reps, RUN &&run, turtle_tuning *best,
+                               double *best_ms)
 {
-    if (!h || !best) return fail(TURTLE_ERR_INVALID, "NULL argument");
-    if (n < 1 || n > h->max_n) return fail(TURTLE_ERR_CAPACITY, "n must be in [1, max_n]");
-    if (with_index && !h->track) return fail(TURTLE_ERR_INVALID, "handle created without index tracking");
-    if (reps < 1) reps = 3;
-    const bool idx = with_index != 0;
     const size_t kbytes = (size_t)n * h->kb, ibytes = (size_t)n * 4;
     void *kin = nullptr, *kout = nullptr;
     uint32_t *iin = nullptr, *iout = nullptr;
@@ -986,9 +1000,9 @@ int turtle_autotune(turtle_handle h, int64_t n, int with_index, int reps, turtle
         cleanup();
         return rc;
     }
-    // keys uniform over the key type's range, iota index
     hipError_t e = ts::fill_keys(h->kb, kin, (uint64_t)n, 0x7e5eed01ull, 0, st);
     if (e == hipSuccess && idx) e = ts::fill_iota(iin, (uint64_t)n, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
         cleanup();
         return fail_hip(e, "autotune fill");
@@ -1006,12 +1020,11 @@ int turtle_autotune(turtle_handle h, int64_t n, int with_index, int reps, turtle
     turtle_tuning best_c = base;
     for (const turtle_tuning &c : cands) {
         h->tune = h->tune_base = c;
-        // one untimed run (tables sized for this tuning), then reps timed runs
-        rc = run_engine(h, kin, iin, kout, iout, (uint64_t)n, nullptr, 1, st);
+        rc = run(kin, iin, kout, iout, st);
         if (!rc) rc = turtle_synchronize(h, nullptr);
         if (rc) break;
         const double t0 = now_s();
-        for (int r = 0; r < reps && !rc; r++) rc = run_engine(h, kin, iin, kout, iout, (uint64_t)n, nullptr, 1, st);
+        for (int r = 0; r < reps && !rc; r++) rc = run(kin, iin, kout, iout, st);
         if (!rc) rc = turtle_synchronize(h, nullptr);
         if (rc) break;
         const double t = (now_s() - t0) / reps;
@@ -1028,6 +1041,41 @@ int turtle_autotune(turtle_handle h, int64_t n, int with_index, int reps, turtle
     *best = best_c;
     if (best_ms) *best_ms = best_t * 1e3;
     return TURTLE_OK;
+}
+
+int turtle_autotune(turtle_handle h, int64_t n, int with_index, int reps, turtle_tuning *best, double *best_ms)
+{
+    if (!h || !best) return fail(TURTLE_ERR_INVALID, "NULL argument");
+    if (n < 1 || n > h->max_n) return fail(TURTLE_ERR_CAPACITY, "n must be in [1, max_n]");
+    if (with_index && !h->track) return fail(TURTLE_ERR_INVALID, "handle created without index tracking");
+    if (reps < 1) reps = 3;
+    const bool idx = with_index != 0;
+    return autotune_candidates(h, n, idx, reps,
+                               [&](void *ki, uint32_t *ii, void *ko, uint32_t *io, hipStream_t st) {
+                                   return run_engine(h, ki, idx ? ii : nullptr, ko, idx ? io : nullptr, (uint64_t)n,
+                                                     nullptr, 1, st);
+                               },
+                               best, best_ms);
+}
+
+int turtle_autotune_segmented(turtle_handle h, const int64_t *offsets, int64_t num_segments, int with_index, int reps,
+                              turtle_tuning *best, double *best_ms)
+{
+    if (!h || !best || !offsets) return fail(TURTLE_ERR_INVALID, "NULL argument");
+    if (num_segments < 1) return fail(TURTLE_ERR_INVALID, "no segments");
+    const int64_t n = offsets[num_segments] - offsets[0];
+    if (offsets[0] != 0) return fail(TURTLE_ERR_INVALID, "offsets must start at 0");
+    if (n < 1 || n > h->max_n) return fail(TURTLE_ERR_CAPACITY, "the segments must hold [1, max_n] keys");
+    if (with_index && !h->track) return fail(TURTLE_ERR_INVALID, "handle created without index tracking");
+    if (reps < 1) reps = 3;
+    const bool idx = with_index != 0;
+    return autotune_candidates(h, n, idx, reps,
+                               [&](void *ki, uint32_t *ii, void *ko, uint32_t *io, hipStream_t st) {
+                                   return turtle_segmented_sort_device(h, ki, idx ? (const int32_t *)ii : nullptr, ko,
+                                                                       idx ? (int32_t *)io : nullptr, offsets,
+                                                                       num_segments, st);
+                               },
+                               best, best_ms);
 }
 
 int turtle_sort_device(turtle_handle h, const void *keys_in, const int32_t *idx_in, void *keys_out, int32_t *idx_out,
@@ -1390,7 +1438,7 @@ int turtle_segmented_graph_create(turtle_handle h, const void *keys_in, const in
     h->host_off = off;   // the level estimate reads the longest segment from here
     // everything the run allocates is allocated before the capture starts:
     // the graph's own tables, sized for its tuning
-    h->tune = tuning_for(h, idx_in != nullptr, longest_root(h, true, (uint64_t)total));
+    h->tune = tuning_for(h, idx_in != nullptr, longest_root(h, true, (uint64_t)total), true);
     rc = ensure_tables(h, h->tune, g->tab, (uint64_t)total, (uint32_t)num_segments, 1);
     if (!rc) rc = ensure_readback(h);
     hipError_t e = hipSuccess;

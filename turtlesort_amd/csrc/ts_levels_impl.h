@@ -1118,7 +1118,7 @@ struct LevelK {
     // their tasks still span ~50 bits for full-range keys, i.e. six 64-bit
     // radix passes, which cost more than the log2(n/SI) merge rounds (the
     // batched 20,000-segment int64 run: 12.7 -> 10.3 ms in the LDS sorts)
-    static constexpr bool MERGE_ONLY = (KB == 8);
+    static constexpr bool MERGE_ONLY = false;
     static constexpr uint32_t CPLAN = MERGE_ONLY ? (uint32_t)C::CMAX + 1 : (uint32_t)CRAD;   // plan's first radix class
     static __device__ __forceinline__ uint32_t lds_class(uint32_t len)
     {
@@ -1148,6 +1148,7 @@ struct LevelK {
         alignas(4) uint16_t cnt[NW][NBIN];   // per-wave digit counts (rank16_add), then the digits' first slots
         U wmin[NW], wmax[NW];             // per-wave key extremes of the step
         uint32_t red[16];
+        uint32_t flag;                    // a group of the step exceeded the group limit
     };
 
     struct PivotShared {
@@ -1293,7 +1294,7 @@ struct LevelK {
         block_sync();
         // every wave derives the step's pass count (the largest key span of
         // its tasks) and its own task's minimum from the per-wave extremes
-        uint32_t maxbits = 0;
+        uint32_t maxbits = 0, tbits = 0;   // the step's widest task span, this wave's task span
         U tmin = 0;
         {
             const uint32_t ntask = rw.nkc + (rw.mixed ? 1u : 0u);
@@ -1308,13 +1309,30 @@ struct LevelK {
                 const uint64_t span = b > a ? (uint64_t)(U)(b - a) : 0ull;
                 const uint32_t bits = span ? (uint32_t)(64 - __clzll(span)) : 0u;
                 maxbits = bits > maxbits ? bits : maxbits;
-                if (a0 == wb) tmin = a;
+                if (a0 == wb) {
+                    tmin = a;
+                    tbits = bits;
+                }
             }
             maxbits = __builtin_amdgcn_readfirstlane(maxbits);
+            tbits = __builtin_amdgcn_readfirstlane(tbits);
             tmin = uniform_val(tmin);
         }
-        uint32_t npass = (maxbits + RW - 1) / RW;
-        npass = npass < MAXPASS ? npass : MAXPASS;
+        uint32_t nfull = (maxbits + RW - 1) / RW;
+        nfull = nfull < MAXPASS ? nfull : MAXPASS;
+        // Spans wider than two digits: the two passes cover the top TOPB
+        // bits of each task's own span (key - min >> lowb, lowb per task);
+        // keys that agree in those bits (a group: neighbours after the
+        // passes) are then ordered by a comparison inside their group.  A
+        // group longer than GMAX (or a group scan that would run past it)
+        // makes the step redo the full passes from its input.  A task whose
+        // span fits the two digits (lowb 0) is sorted exactly by them.
+        // (8-byte keys: their leaves span ~50 bits; narrower keys keep the
+        // registers of the plain passes)
+        constexpr uint32_t TOPP = 2, TOPB = TOPP * RW, GMAX = 64;
+        bool top = KB == 8 && nfull > TOPP;   // block-uniform
+        uint32_t lowb = top && tbits > TOPB ? tbits - TOPB : 0u;   // wave-uniform
+        uint32_t npass = top ? TOPP : nfull;
         // scan ownership: the step's entries in (task, digit, wave of the
         // task) order are NBIN per wave in use, so thread tid owns the EPT
         // consecutive entries from tid*EPT, all inside its own wave's task:
@@ -1338,95 +1356,173 @@ struct LevelK {
                 }
             }
         }
-        for (uint32_t pass = 0; pass < npass; pass++) {
-            const uint32_t sh = pass * RW;
-            if (pass > 0 && act) {
+        U *dst = (U *)p.kout;
+        const BufRsrc kw = buf_rsrc(dst + rw.tk.begin, len * (uint32_t)KB);
+        const BufRsrc iw = buf_rsrc(p.iout + rw.tk.begin, IDX ? len * 4u : 0u);
+        for (uint32_t attempt = 0; attempt < 2; attempt++) {   // block-uniform
+            if (attempt == 1) {
+                // the full passes, from the task's input again
+                const BufRsrc kr = buf_rsrc(src_keys<U>(p, rw.tk.depth) + rw.tk.begin, len * (uint32_t)KB);
+                const BufRsrc ir = buf_rsrc(src_idx(p, rw.tk.depth) + rw.tk.begin, IDX ? len * 4u : 0u);
+#pragma unroll
+                for (int i = 0; i < SI; i++) k[i] = buf_load<U>(kr, (j0 + (uint32_t)i * WAVE) * (uint32_t)KB);
+#pragma unroll
+                for (int i = 0; i < SI; i++) v[i] = IDX ? buf_load<uint32_t>(ir, (j0 + (uint32_t)i * WAVE) * 4u) : 0u;
 #pragma unroll
                 for (int i = 0; i < SI; i++) {
-                    k[i] = rk[i * WAVE];
-                    if (IDX) v[i] = ri[i * WAVE];
-                }
-            }
-            uint32_t dr[SI];
-            if (act) {
-                for (uint32_t d = l; d < NBIN; d += WAVE) rs.cnt[w][d] = 0;
-                wave_sync();
-                // after a pass the task's keys are at its first len positions
-                auto digit = [&](int i) {
                     const bool real = j0 + (uint32_t)i * WAVE < len;
-                    return real ? (uint32_t)((U)(k[i] - tmin) >> sh) & (NBIN - 1) : NBIN - 1;
-                };
-                // an item whose lanes share one digit (runs of sorted keys)
-                // takes its ranks with one atomic (64 same-address atomics
-                // are serialised by the LDS); tested per item only when the
-                // wave's first or last item is such one
-                auto same = [&](uint32_t d) {
-                    const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
-                    return ballot(d == d0) == ~0ull;
-                };
-                const bool runs = same(digit(0)) || same(digit(SI - 1));   // wave-uniform
-                if (runs) {
-#pragma unroll
-                    for (int i = 0; i < SI; i++) {
-                        const uint32_t d = digit(i);
-                        uint32_t r;
-                        if (same(d)) {   // wave-uniform
-                            uint32_t o0 = 0;
-                            if (l == 0) o0 = rank16_add((uint32_t *)rs.cnt[w], d, (uint32_t)WAVE);
-                            r = __builtin_amdgcn_readlane(o0, 0) + l;
-                        } else {
-                            r = rank16_add((uint32_t *)rs.cnt[w], d, 1u);
-                        }
-                        dr[i] = d | (r << 16);
-                    }
-                } else {
-#pragma unroll
-                    for (int i = 0; i < SI; i++) {
-                        const uint32_t d = digit(i);
-                        dr[i] = d | (rank16_add((uint32_t *)rs.cnt[w], d, 1u) << 16);
-                    }
+                    k[i] = real ? to_ord<U, FL>(k[i], flip) : key_max<U>();
                 }
             }
-            block_sync();
-            // exclusive scan over (task, digit, wave of the task): the first
-            // slot of every (wave, digit) run of the step
-            {
-                uint32_t cv[EPT], loc = 0;
+            for (uint32_t pass = 0; pass < npass; pass++) {
+                const uint32_t sh = lowb + pass * RW;
+                if (pass > 0 && act) {
 #pragma unroll
-                for (int k2 = 0; k2 < (int)EPT; k2++) {
-                    cv[k2] = 0u;
-                    if (act)   // wave-uniform
-                        cv[k2] = rs.cnt[wb + ((ew >> (4 * k2)) & 15u)][d0 + ((ej >> (4 * k2)) & 15u)];
-                    loc += cv[k2];
+                    for (int i = 0; i < SI; i++) {
+                        k[i] = rk[i * WAVE];
+                        if (IDX) v[i] = ri[i * WAVE];
+                    }
                 }
-                uint32_t tot;
-                uint32_t run = block_excl_scan<NT>(loc, rs.red, &tot);
+                uint32_t dr[SI];
                 if (act) {
+                    for (uint32_t d = l; d < NBIN; d += WAVE) rs.cnt[w][d] = 0;
+                    wave_sync();
+                    // after a pass the task's keys are at its first len positions
+                    auto digit = [&](int i) {
+                        const bool real = j0 + (uint32_t)i * WAVE < len;
+                        return real ? (uint32_t)((U)(k[i] - tmin) >> sh) & (NBIN - 1) : NBIN - 1;
+                    };
+                    // an item whose lanes share one digit (runs of sorted keys)
+                    // takes its ranks with one atomic (64 same-address atomics
+                    // are serialised by the LDS); tested per item only when the
+                    // wave's first or last item is such one
+                    auto same = [&](uint32_t d) {
+                        const uint32_t dd = __builtin_amdgcn_readfirstlane(d);
+                        return ballot(d == dd) == ~0ull;
+                    };
+                    const bool runs = same(digit(0)) || same(digit(SI - 1));   // wave-uniform
+                    if (runs) {
+#pragma unroll
+                        for (int i = 0; i < SI; i++) {
+                            const uint32_t d = digit(i);
+                            uint32_t r;
+                            if (same(d)) {   // wave-uniform
+                                uint32_t o0 = 0;
+                                if (l == 0) o0 = rank16_add((uint32_t *)rs.cnt[w], d, (uint32_t)WAVE);
+                                r = __builtin_amdgcn_readlane(o0, 0) + l;
+                            } else {
+                                r = rank16_add((uint32_t *)rs.cnt[w], d, 1u);
+                            }
+                            dr[i] = d | (r << 16);
+                        }
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < SI; i++) {
+                            const uint32_t d = digit(i);
+                            dr[i] = d | (rank16_add((uint32_t *)rs.cnt[w], d, 1u) << 16);
+                        }
+                    }
+                }
+                block_sync();
+                // exclusive scan over (task, digit, wave of the task): the first
+                // slot of every (wave, digit) run of the step
+                {
+                    uint32_t cv[EPT], loc = 0;
 #pragma unroll
                     for (int k2 = 0; k2 < (int)EPT; k2++) {
-                        rs.cnt[wb + ((ew >> (4 * k2)) & 15u)][d0 + ((ej >> (4 * k2)) & 15u)] = (uint16_t)run;
-                        run += cv[k2];
+                        cv[k2] = 0u;
+                        if (act)   // wave-uniform
+                            cv[k2] = rs.cnt[wb + ((ew >> (4 * k2)) & 15u)][d0 + ((ej >> (4 * k2)) & 15u)];
+                        loc += cv[k2];
+                    }
+                    uint32_t tot;
+                    uint32_t run = block_excl_scan<NT>(loc, rs.red, &tot);
+                    if (act) {
+#pragma unroll
+                        for (int k2 = 0; k2 < (int)EPT; k2++) {
+                            rs.cnt[wb + ((ew >> (4 * k2)) & 15u)][d0 + ((ej >> (4 * k2)) & 15u)] = (uint16_t)run;
+                            run += cv[k2];
+                        }
                     }
                 }
+                block_sync();
+                if (act) {
+#pragma unroll
+                    for (int i = 0; i < SI; i++) {
+                        const uint32_t pos = (rs.cnt[w][dr[i] & 0xffffu] + (dr[i] >> 16)) & (CAP - 1);
+                        sk[pos] = k[i];
+                        if (IDX) si[pos] = v[i];
+                    }
+                }
+                block_sync();
             }
-            block_sync();
+            if (KB != 8 || !top) break;
+            // groups: a key whose neighbour (task order) has the same top
+            // bits takes its rank inside the group by comparisons (ties by
+            // position, so the order stays stable)
+            const U *tk = sk + wb * WS;   // the task's keys in their sorted slots
+            uint32_t pos[SI];
+            bool grp[SI], ovf = false, any = false;
             if (act) {
 #pragma unroll
                 for (int i = 0; i < SI; i++) {
-                    const uint32_t pos = (rs.cnt[w][dr[i] & 0xffffu] + (dr[i] >> 16)) & (CAP - 1);
-                    sk[pos] = k[i];
-                    if (IDX) si[pos] = v[i];
+                    const uint32_t j = j0 + (uint32_t)i * WAVE;
+                    k[i] = rk[i * WAVE];
+                    if (lowb == 0) {   // wave-uniform: the task is sorted
+                        grp[i] = false;
+                        pos[i] = j;
+                        continue;
+                    }
+                    const U pf = (U)((U)(k[i] - tmin) >> lowb);
+                    const U pl = j > 0 && j <= len ? (U)((U)(tk[j - 1] - tmin) >> lowb) : (U)~pf;
+                    const U pr = j + 1 < len ? (U)((U)(tk[j + 1] - tmin) >> lowb) : (U)~pf;
+                    grp[i] = j < len && (pl == pf || pr == pf);
+                    pos[i] = j;
+                    any |= grp[i];
+                }
+                if (ballot(any)) {   // wave-uniform: rare for spread keys
+#pragma unroll
+                    for (int i = 0; i < SI; i++) {
+                        if (grp[i]) {
+                            const uint32_t j = j0 + (uint32_t)i * WAVE;
+                            const U pf = (U)((U)(k[i] - tmin) >> lowb);
+                            uint32_t gs = j, ge = j + 1;
+                            while (gs > 0 && j - gs < GMAX && (U)((U)(tk[gs - 1] - tmin) >> lowb) == pf) gs--;
+                            while (ge < len && ge - j < GMAX && (U)((U)(tk[ge] - tmin) >> lowb) == pf) ge++;
+                            ovf |= j - gs >= GMAX || ge - j >= GMAX;
+                            uint32_t r = 0;
+                            for (uint32_t t = gs; t < ge; t++) {
+                                const U kt = tk[t];
+                                r += (kt < k[i] || (kt == k[i] && t < j)) ? 1u : 0u;
+                            }
+                            pos[i] = gs + r;
+                        }
+                    }
                 }
             }
+            if (tid == 0) rs.flag = 0;
             block_sync();
+            if (ovf) rs.flag = 1;   // every writer stores the same value
+            block_sync();
+            if (__builtin_amdgcn_readfirstlane(rs.flag) == 0) {
+                if (act) {
+#pragma unroll
+                    for (int i = 0; i < SI; i++) {
+                        buf_store<U>(kw, pos[i] * (uint32_t)KB, from_ord<U, FL>(k[i], flip));
+                        if (IDX) buf_store<uint32_t>(iw, pos[i] * 4u, ri[i * WAVE]);
+                    }
+                }
+                block_sync();   // the next step rewrites the LDS arrays
+                return;
+            }
+            top = false;
+            lowb = 0;
+            npass = nfull;
         }
         // write-out of the wave's slots: slot w*WS + i*WAVE + l holds task
         // position j0 + i*WAVE (every task's keys sit at the start of its
         // slots; with no pass, they are still in registers in load order)
         if (act) {
-            U *dst = (U *)p.kout;
-            const BufRsrc kw = buf_rsrc(dst + rw.tk.begin, len * (uint32_t)KB);
-            const BufRsrc iw = buf_rsrc(p.iout + rw.tk.begin, IDX ? len * 4u : 0u);
             if (npass > 0) {
 #pragma unroll
                 for (int i = 0; i < SI; i++) {
@@ -1915,7 +2011,9 @@ hipError_t run_levels(const Params &p, int cus, int wg_per_cu, const uint32_t *d
     }();
     const int o_tile = oc.tile, o_cnt = oc.cnt, o_lds = oc.lds;
     auto lim = [&](int o) { return cus * ((wg_per_cu > 0 && wg_per_cu < o) ? wg_per_cu : o); };
-    const int g_tile = lim(o_tile), g_cnt = lim(o_cnt), g_pivot = cus, g_lds = cus * o_lds;
+    // (two pivot workgroups per CU: a level of many short segments waits on
+    // the sample loads)
+    const int g_tile = lim(o_tile), g_cnt = lim(o_cnt), g_pivot = 2 * cus, g_lds = cus * o_lds;
     const int g_small = cus * 2;
 #define TS_LAUNCH(kind, ...)                  \
     do {                                      \
