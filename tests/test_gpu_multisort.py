@@ -118,3 +118,60 @@ def test_multisort_index_device(orc, torch_dev, arr):
     np.testing.assert_array_equal(d2h(torch, iout2).reshape(-1, arr),
                                   np.take_along_axis(d2h(torch, pay).reshape(-1, arr), perm, axis=1))
     sorter.close()
+
+
+# ------------------------------------------- arrays of 513..1024 4-byte keys
+def _radix_rows(kind, arr, num, dtype, seed):
+    """rows that steer the one-wave LDS radix of 513..1024-key arrays: the
+    top 18 bits of (key - min) first, keys sharing those bits ordered inside
+    their group, and the exact 9-bit passes when a group exceeds 64 keys or
+    the span fits 18 bits"""
+    rng = np.random.default_rng(seed)
+    ui = np.dtype(f"u{np.dtype(dtype).itemsize}")
+    spread = rng.integers(0, 1 << 32, size=(num, arr), dtype=np.uint64)
+    if kind == "spread":
+        rows = spread
+    elif kind == "pairs":            # many keys share their top bits with a neighbour
+        rows = spread.copy()
+        rows[:, 1::2] = (rows[:, 0::2][:, : arr // 2] & ~np.uint64(0x3FFF)) | rng.integers(
+            0, 1 << 14, size=(num, arr // 2), dtype=np.uint64)
+    elif kind in ("group64", "group65", "group200"):   # one group of that size
+        g = int(kind[5:])
+        rows = spread.copy()
+        base = rng.integers(0, 1 << 31, size=(num, 1), dtype=np.uint64) & ~np.uint64(0x3FFF)
+        perm = rng.permuted(np.tile(np.arange(arr), (num, 1)), axis=1)
+        vals = base + rng.integers(0, 1 << 13, size=(num, g), dtype=np.uint64)
+        np.put_along_axis(rows, perm[:, :g], vals, axis=1)
+        # the span covers 32 bits (two keys outside the group)
+        np.put_along_axis(rows, perm[:, g:g + 2], np.array([[0, 0xFFFFFFFF]], dtype=np.uint64), axis=1)
+    elif kind in ("dups", "dups_big"):   # equal keys: ties decided by position
+        rows = rng.integers(0, 40 if kind == "dups" else 8, size=(num, arr), dtype=np.uint64) << np.uint64(26)
+    elif kind == "narrow":            # span of 17 bits: exact passes
+        rows = rng.integers(0, 1 << 17, size=(num, arr), dtype=np.uint64) + np.uint64(123456789)
+    else:
+        raise ValueError(kind)
+    return rows.astype(np.uint32).view(ui).view(dtype).reshape(-1)
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.uint32, np.float32])
+@pytest.mark.parametrize("kind", ["spread", "pairs", "group64", "group65", "group200", "dups", "dups_big", "narrow"])
+def test_multisort_radix_groups(orc, torch_dev, dtype, kind):
+    torch = torch_dev
+    ui = np.dtype("u4")
+    sorter = TurtleSort(dtype, 1024 * 300)
+    for arr in (513, 700, 1000, 1024):
+        num = 203 if arr < 1024 else 300
+        print(f"case radix-groups dtype={np.dtype(dtype)} kind={kind} arr={arr} num={num}", flush=True)
+        d = _radix_rows(kind, arr, num, dtype, arr * 31 + len(kind))
+        ek, perm = row_expect(orc, d, arr)
+        kin = h2d(torch, d.view(np.int32))
+        kout = torch.empty_like(kin)
+        sorter.multi_sort_device(arr, kin, kout)
+        sorter.synchronize()
+        np.testing.assert_array_equal(d2h(torch, kout).view(ui), ek.view(ui))
+        iout = torch.empty(d.size, dtype=torch.int32, device="cuda")
+        sorter.multi_sort_index_device(arr, kin, kout, iout)
+        sorter.synchronize()
+        np.testing.assert_array_equal(d2h(torch, kout).view(ui), ek.view(ui))
+        np.testing.assert_array_equal(d2h(torch, iout).reshape(-1, arr), perm)
+    sorter.close()

@@ -227,14 +227,22 @@ __global__ void __launch_bounds__(NT) ms_small(const U *__restrict__ in, U *__re
 
 // ---- arrays of 513..1024 keys of up to 4 bytes: one wave per array, LDS radix ------
 // Each wave sorts one array (four waves per workgroup, no block barrier):
-// its keys are loaded into registers (16 per lane, lane l holding positions
-// i*64 + l), and stable LSD passes of 8-bit digits over the key bits that
-// differ within the array sort them: per pass a per-wave counter per digit
-// gives every key its rank with one LDS atomic (rank16_add: the wave's
-// keys of a digit in (item, lane) order, i.e. in position order), a
-// 64-lane scan of the 256 counters gives each digit's first slot, and the
-// keys are placed in LDS and read back in position order.  Positions past
-// the array's end hold the largest key and stay behind its keys.
+// its keys are staged in LDS in position order and sorted by stable LSD
+// passes over (key - min): per pass each lane reads 16 keys (positions
+// i*64 + l), a per-wave counter per 9-bit digit gives every key its rank
+// with one LDS atomic (rank16_add: the wave's keys of a digit in (item,
+// lane) order, i.e. in position order), a 64-lane scan of the counters
+// gives each digit's first slot, and the keys are placed in LDS.  Positions
+// past the array's end hold the largest key and stay behind its keys.
+//
+// An array whose key span (max - min) is wider than two 9-bit digits is
+// sorted by its top 18 span bits only ((key - min) >> lowb, two passes);
+// keys that then agree in those bits (a group; for 1024 spread keys about
+// two pairs per array) are neighbours, found by comparing each key with
+// the one before it across lanes; the lane holding a group's first
+// position orders the group in LDS by insertion sort (stable).  A group
+// longer than GMAX makes the array take the exact passes instead (as does
+// a span of at most 18 bits): 9-bit digits over the whole span.
 template <typename U, bool FL, bool IDX>
 __global__ void __launch_bounds__(256) ms_radix(const U *__restrict__ in, U *__restrict__ out,
                                                  const uint32_t *__restrict__ iin, uint32_t *__restrict__ iout,
@@ -243,88 +251,207 @@ __global__ void __launch_bounds__(256) ms_radix(const U *__restrict__ in, U *__r
     static_assert(sizeof(U) <= 4, "keys of up to 4 bytes");
     constexpr int WPB = 4, IT = 16;
     constexpr uint32_t NS = WAVE * IT;       // 1024 slots per wave
+    constexpr uint32_t RB = 9, NBIN = 1u << RB, TOPB = 2 * RB, GMAX = 64;
     __shared__ U sk[WPB][NS];
     __shared__ uint32_t si[IDX ? WPB : 1][IDX ? NS : 1];
-    __shared__ uint32_t cnt[WPB][128];       // 256 16-bit digit counters per wave
+    __shared__ uint32_t cnt[WPB][NBIN / 2];  // 16-bit digit counters per wave
     const uint32_t w = threadIdx.x / WAVE, l = lane();
     const uint64_t arr = (uint64_t)blockIdx.x * WPB + w;
     if (arr >= num_arrays) return;           // wave-uniform; no block barrier below
     const size_t base = (size_t)arr * arr_size;
-    U k[IT];
-    uint32_t v[IT];
-#pragma unroll
-    for (int i = 0; i < IT; i++) {
-        const uint32_t j = (uint32_t)i * WAVE + l;
-        k[i] = j < arr_size ? in[base + j] : (U)0;
-        v[i] = IDX && iin && j < arr_size ? iin[base + j] : j;
-    }
-    // bits that differ among the array's keys: a pass whose digit is the
-    // same for every key is skipped
-    U band = key_max<U>(), bor = 0;
-#pragma unroll
-    for (int i = 0; i < IT; i++) {
-        const uint32_t j = (uint32_t)i * WAVE + l;
-        k[i] = j < arr_size ? to_ord<U, FL>(k[i], flip) : key_max<U>();
-        if (j < arr_size) {
-            band &= k[i];
-            bor |= k[i];
-        }
-    }
-#pragma unroll
-    for (int d = WAVE / 2; d >= 1; d >>= 1) {
-        band &= (U)__shfl_xor((uint32_t)band, d, WAVE);
-        bor |= (U)__shfl_xor((uint32_t)bor, d, WAVE);
-    }
-    const uint32_t diff = (uint32_t)(band ^ bor);
     U *const ks = sk[w];
     uint32_t *const vs = si[IDX ? w : 0];
     uint32_t *const cw = cnt[w];
-    for (uint32_t sh = 0; sh < 8 * sizeof(U); sh += 8) {
-        if (((diff >> sh) & 0xffu) == 0) continue;   // wave-uniform
-        cw[l] = 0;
-        cw[l + WAVE] = 0;
-        wave_sync();
-        uint32_t dr[IT];
+    // load: keys (order representation) and indices into LDS in position
+    // order; positions past the array's end hold the largest key
+    auto load = [&]() {
+        uint32_t l = lane();
+        asm volatile("" : "+v"(l));
+        U k[IT];
+        uint32_t v[IT];
 #pragma unroll
         for (int i = 0; i < IT; i++) {
-            const uint32_t d = ((uint32_t)k[i] >> sh) & 0xffu;
-            dr[i] = d | (rank16_add(cw, d, 1u) << 8);
+            const uint32_t j = (uint32_t)i * WAVE + l;
+            k[i] = j < arr_size ? in[base + j] : (U)0;
+            v[i] = IDX && iin && j < arr_size ? iin[base + j] : j;
         }
-        wave_sync();
-        // exclusive scan of the 256 counters: lane l owns digits 4l .. 4l+3
-        // (the two halves of words 2l and 2l+1)
-        {
-            const uint32_t w0 = cw[2 * l], w1 = cw[2 * l + 1];
-            const uint32_t c0 = w0 & 0xffffu, c1 = w0 >> 16, c2 = w1 & 0xffffu, c3 = w1 >> 16;
-            const uint32_t tot = c0 + c1 + c2 + c3;
-            const uint32_t p0 = wave_incl_scan(tot) - tot, p1 = p0 + c0, p2 = p1 + c1, p3 = p2 + c2;
-            cw[2 * l] = p0 | (p1 << 16);
-            cw[2 * l + 1] = p2 | (p3 << 16);
-        }
-        wave_sync();
 #pragma unroll
         for (int i = 0; i < IT; i++) {
-            const uint32_t d = dr[i] & 0xffu;
-            const uint32_t first = (cw[d >> 1] >> ((d & 1u) << 4)) & 0xffffu;
-            const uint32_t pos = (first + (dr[i] >> 8)) & (NS - 1);
-            ks[pos] = k[i];
-            if (IDX) vs[pos] = v[i];
+            const uint32_t j = (uint32_t)i * WAVE + l;
+            ks[j] = j < arr_size ? to_ord<U, FL>(k[i], flip) : key_max<U>();
+            if (IDX) vs[j] = v[i];
         }
         wave_sync();
+    };
+    // one stable pass: digit(i) of every item, ranks, 64-lane scan of the
+    // NB counters, placement in LDS, read-back in position order
+    auto pass = [&](auto digit, auto nbc) {
+        constexpr uint32_t nb = decltype(nbc)::value;
+        uint32_t l = lane();
+        asm volatile("" : "+v"(l));   // keeps per-lane addresses out of the enclosing loops
+        U k[IT];
+        uint32_t v[IT];
 #pragma unroll
         for (int i = 0; i < IT; i++) {
             k[i] = ks[(uint32_t)i * WAVE + l];
             if (IDX) v[i] = vs[(uint32_t)i * WAVE + l];
         }
+        for (uint32_t c = l; c < nb / 2; c += WAVE) cw[c] = 0;
         wave_sync();
-    }
+        uint32_t dr[IT];
+#pragma unroll
+        for (int i = 0; i < IT; i++) {
+            const uint32_t d = digit(k[i]);
+            dr[i] = d | (rank16_add(cw, d, 1u) << 16);
+        }
+        wave_sync();
+        // exclusive scan: lane l owns the nb/64 consecutive digits from l*nb/64
+        {
+            constexpr uint32_t per = nb / WAVE / 2;   // counter words per lane (2 or 4)
+            uint32_t c[2 * per], tot = 0;
+#pragma unroll
+            for (uint32_t q = 0; q < per; q++) {
+                const uint32_t wd = cw[l * per + q];
+                c[2 * q] = wd & 0xffffu;
+                c[2 * q + 1] = wd >> 16;
+                tot += c[2 * q] + c[2 * q + 1];
+            }
+            uint32_t run = wave_incl_scan(tot) - tot;
+#pragma unroll
+            for (uint32_t q = 0; q < per; q++) {
+                const uint32_t p0 = run, p1 = run + c[2 * q];
+                run = p1 + c[2 * q + 1];
+                cw[l * per + q] = p0 | (p1 << 16);
+            }
+        }
+        wave_sync();
+#pragma unroll
+        for (int i = 0; i < IT; i++) {
+            const uint32_t d = dr[i] & 0xffffu;
+            const uint32_t first = (cw[d >> 1] >> ((d & 1u) << 4)) & 0xffffu;
+            const uint32_t pos = (first + (dr[i] >> 16)) & (NS - 1);
+            ks[pos] = k[i];
+            if (IDX) vs[pos] = v[i];
+        }
+        wave_sync();
+    };
+    load();
+    U mn = key_max<U>(), mx = 0;
 #pragma unroll
     for (int i = 0; i < IT; i++) {
         const uint32_t j = (uint32_t)i * WAVE + l;
         if (j < arr_size) {
-            out[base + j] = from_ord<U, FL>(k[i], flip);
-            if (IDX) iout[base + j] = v[i];
+            const U key = ks[j];
+            mn = key < mn ? key : mn;
+            mx = key > mx ? key : mx;
         }
+    }
+#pragma unroll
+    for (int d = WAVE / 2; d >= 1; d >>= 1) {
+        const U omn = (U)__shfl_xor((uint32_t)mn, d, WAVE), omx = (U)__shfl_xor((uint32_t)mx, d, WAVE);
+        mn = omn < mn ? omn : mn;
+        mx = omx > mx ? omx : mx;
+    }
+    const uint32_t span = (uint32_t)(U)(mx - mn);
+    const uint32_t bits = span ? 32u - (uint32_t)__clz(span) : 0u;
+    const uint32_t nfull = (bits + RB - 1) / RB;
+    bool top = nfull > 2;   // wave-uniform
+    uint32_t lowb = top ? bits - TOPB : 0u;
+    // offset from the minimum, clamped to the span: padding positions (the
+    // largest key) tie with the array's maximum and, being later, stay behind
+    auto rel = [&](U key) {
+        const uint32_t r = (uint32_t)(U)(key - mn);
+        return r < span ? r : span;
+    };
+    auto top_of = [&](U key) { return rel(key) >> lowb; };
+#pragma nounroll
+    for (uint32_t attempt = 0; attempt < 2; attempt++) {   // wave-uniform
+        if (attempt == 1) {
+            wave_sync();
+            load();
+        }
+        const uint32_t npass = top ? 2u : nfull;
+#pragma nounroll
+        for (uint32_t p = 0; p < npass; p++) {
+            const uint32_t sh = lowb + p * RB;
+            pass([&](U key) { return (rel(key) >> sh) & (NBIN - 1); },
+                 std::integral_constant<uint32_t, NBIN>());
+        }
+        if (top) {
+            // groups: runs of keys with equal top bits (position order
+            // j = i*64 + l; the key before j is lane l-1 of item i, or lane 63
+            // of item i-1; padding positions hold the largest key and never
+            // join).  eq bit i: the key before item i's has the same top bits.
+            // The neighbour comes from a DPP whole-wave shift by one lane,
+            // lane 0 taking lane 63 of the item before.
+            uint32_t l = lane();
+            asm volatile("" : "+v"(l));
+            uint32_t eq = 0, last = 0;
+#pragma unroll
+            for (int i = 0; i < IT; i++) {
+                const uint32_t t = top_of(ks[(uint32_t)i * WAVE + l]);
+                const uint32_t before = (uint32_t)__builtin_amdgcn_update_dpp((int)last, (int)t, 0x138, 0xf, 0xf, false);
+                const uint32_t j = (uint32_t)i * WAVE + l;
+                if (j < arr_size && j > 0 && before == t) eq |= 1u << i;
+                last = (uint32_t)__builtin_amdgcn_readlane((int)t, WAVE - 1);
+            }
+            // a group's head has no eq bit and the key after it has one (lane
+            // l+1 of item i, or lane 0 of item i+1: the shift the other way)
+            const uint32_t eq_first = (uint32_t)__builtin_amdgcn_readfirstlane((int)eq);
+            const uint32_t after = (uint32_t)__builtin_amdgcn_update_dpp((int)(eq_first >> 1), (int)eq, 0x130, 0xf, 0xf,
+                                                                         false);
+            const uint32_t heads = after & ~eq;
+            bool big = false;
+            if (ballot(heads != 0)) {   // wave-uniform: rare for spread keys
+                // the head's lane orders its group in LDS by insertion sort
+                // (stable); a group longer than GMAX makes the array take the
+                // exact passes
+#pragma nounroll
+                for (uint32_t g = heads; g; g &= g - 1) {
+                    const uint32_t gs = (uint32_t)__builtin_ctz(g) * WAVE + l;
+                    const uint32_t t = top_of(ks[gs]);
+                    uint32_t ge = gs + 1;
+#pragma nounroll
+                    while (ge < arr_size && ge - gs <= GMAX && top_of(ks[ge]) == t) ge++;
+                    if (ge - gs > GMAX) {
+                        big = true;
+                        break;
+                    }
+#pragma nounroll
+                    for (uint32_t q = gs + 1; q < ge; q++) {
+                        const U kq = ks[q];
+                        const uint32_t vq = IDX ? vs[q] : 0u;
+                        uint32_t r = q;
+#pragma nounroll
+                        while (r > gs && ks[r - 1] > kq) {
+                            ks[r] = ks[r - 1];
+                            if (IDX) vs[r] = vs[r - 1];
+                            r--;
+                        }
+                        ks[r] = kq;
+                        if (IDX) vs[r] = vq;
+                    }
+                }
+            }
+            if (ballot(big)) {   // wave-uniform
+                top = false;
+                lowb = 0;
+                continue;
+            }
+            wave_sync();
+        }
+        // write-out in position order from LDS
+        uint32_t l = lane();
+        asm volatile("" : "+v"(l));
+#pragma unroll
+        for (int i = 0; i < IT; i++) {
+            const uint32_t j = (uint32_t)i * WAVE + l;
+            if (j < arr_size) {
+                out[base + j] = from_ord<U, FL>(ks[j], flip);
+                if (IDX) iout[base + j] = vs[j];
+            }
+        }
+        break;
     }
 }
 
