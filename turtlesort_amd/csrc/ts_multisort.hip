@@ -250,45 +250,59 @@ __global__ void __launch_bounds__(256) ms_radix(const U *__restrict__ in, U *__r
 {
     static_assert(sizeof(U) <= 4, "keys of up to 4 bytes");
     constexpr int WPB = 4, IT = 16;
+    constexpr int IT_FULL = 8;               // arrays hold more than 512 keys: items 0..7 always valid
     constexpr uint32_t NS = WAVE * IT;       // 1024 slots per wave
     constexpr uint32_t RB = 9, NBIN = 1u << RB, TOPB = 2 * RB, GMAX = 64;
+    constexpr uint32_t CPL = NBIN / WAVE;    // counters per lane in the scan
     __shared__ U sk[WPB][NS];
     __shared__ uint32_t si[IDX ? WPB : 1][IDX ? NS : 1];
-    __shared__ uint32_t cnt[WPB][NBIN / 2];  // 16-bit digit counters per wave
-    const uint32_t w = threadIdx.x / WAVE, l = lane();
+    __shared__ uint32_t cnt[WPB][NBIN];      // digit counters per wave
+    const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
     const uint64_t arr = (uint64_t)blockIdx.x * WPB + w;
     if (arr >= num_arrays) return;           // wave-uniform; no block barrier below
     const size_t base = (size_t)arr * arr_size;
+    // range-checked views of this array: loads past its end read 0, stores
+    // past it are dropped
+    const BufRsrc rin = buf_rsrc(in + base, arr_size * (uint32_t)sizeof(U));
+    const BufRsrc rout = buf_rsrc(out + base, arr_size * (uint32_t)sizeof(U));
+    const BufRsrc riin = buf_rsrc(IDX && iin ? iin + base : nullptr, IDX && iin ? arr_size * 4u : 0u);
+    const BufRsrc riout = buf_rsrc(IDX ? iout + base : nullptr, IDX ? arr_size * 4u : 0u);
     U *const ks = sk[w];
     uint32_t *const vs = si[IDX ? w : 0];
     uint32_t *const cw = cnt[w];
     // load: keys (order representation) and indices into LDS in position
-    // order; positions past the array's end hold the largest key
-    auto load = [&]() {
+    // order; positions past the array's end hold the largest key.  Returns
+    // the lane's smallest and largest key.
+    auto load = [&](U &mn, U &mx) {
         uint32_t l = lane();
-        asm volatile("" : "+v"(l));
+        asm volatile("" : "+v"(l));   // keeps per-lane addresses out of the enclosing loops
         U k[IT];
         uint32_t v[IT];
 #pragma unroll
         for (int i = 0; i < IT; i++) {
             const uint32_t j = (uint32_t)i * WAVE + l;
-            k[i] = j < arr_size ? in[base + j] : (U)0;
-            v[i] = IDX && iin && j < arr_size ? iin[base + j] : j;
+            k[i] = buf_load<U>(rin, j * (uint32_t)sizeof(U));
+            if (IDX) v[i] = iin ? buf_load<uint32_t>(riin, j * 4u) : j;
         }
+        mn = key_max<U>();
+        mx = 0;
 #pragma unroll
         for (int i = 0; i < IT; i++) {
             const uint32_t j = (uint32_t)i * WAVE + l;
-            ks[j] = j < arr_size ? to_ord<U, FL>(k[i], flip) : key_max<U>();
+            const bool ok = i < IT_FULL || j < arr_size;
+            const U key = to_ord<U, FL>(k[i], flip);
+            mn = ok && key < mn ? key : mn;
+            mx = ok && key > mx ? key : mx;
+            ks[j] = ok ? key : key_max<U>();
             if (IDX) vs[j] = v[i];
         }
         wave_sync();
     };
-    // one stable pass: digit(i) of every item, ranks, 64-lane scan of the
-    // NB counters, placement in LDS, read-back in position order
-    auto pass = [&](auto digit, auto nbc) {
-        constexpr uint32_t nb = decltype(nbc)::value;
+    // one stable pass: digit(key) of the 16 items each lane reads in position
+    // order, ranks through the counters, 64-lane scan, placement in LDS
+    auto pass = [&](auto digit) {
         uint32_t l = lane();
-        asm volatile("" : "+v"(l));   // keeps per-lane addresses out of the enclosing loops
+        asm volatile("" : "+v"(l));
         U k[IT];
         uint32_t v[IT];
 #pragma unroll
@@ -296,63 +310,51 @@ __global__ void __launch_bounds__(256) ms_radix(const U *__restrict__ in, U *__r
             k[i] = ks[(uint32_t)i * WAVE + l];
             if (IDX) v[i] = vs[(uint32_t)i * WAVE + l];
         }
-        for (uint32_t c = l; c < nb / 2; c += WAVE) cw[c] = 0;
+#pragma unroll
+        for (uint32_t c = 0; c < CPL; c++) cw[l * CPL + c] = 0;
         wave_sync();
-        uint32_t dr[IT];
+        uint32_t d[IT], r[IT];
 #pragma unroll
         for (int i = 0; i < IT; i++) {
-            const uint32_t d = digit(k[i]);
-            dr[i] = d | (rank16_add(cw, d, 1u) << 16);
+            d[i] = digit(k[i]);
+            r[i] = atomicAdd(&cw[d[i]], 1u);   // stable rank (see rank16_add)
         }
         wave_sync();
-        // exclusive scan: lane l owns the nb/64 consecutive digits from l*nb/64
+        // exclusive scan: lane l owns the CPL consecutive digits from l*CPL
         {
-            constexpr uint32_t per = nb / WAVE / 2;   // counter words per lane (2 or 4)
-            uint32_t c[2 * per], tot = 0;
+            uint32_t c[CPL], tot = 0;
 #pragma unroll
-            for (uint32_t q = 0; q < per; q++) {
-                const uint32_t wd = cw[l * per + q];
-                c[2 * q] = wd & 0xffffu;
-                c[2 * q + 1] = wd >> 16;
-                tot += c[2 * q] + c[2 * q + 1];
+            for (uint32_t q = 0; q < CPL; q++) {
+                c[q] = cw[l * CPL + q];
+                tot += c[q];
             }
             uint32_t run = wave_incl_scan(tot) - tot;
 #pragma unroll
-            for (uint32_t q = 0; q < per; q++) {
-                const uint32_t p0 = run, p1 = run + c[2 * q];
-                run = p1 + c[2 * q + 1];
-                cw[l * per + q] = p0 | (p1 << 16);
+            for (uint32_t q = 0; q < CPL; q++) {
+                cw[l * CPL + q] = run;
+                run += c[q];
             }
         }
         wave_sync();
+        // the counts add up to the wave's 1024 slots, so pos < NS
 #pragma unroll
         for (int i = 0; i < IT; i++) {
-            const uint32_t d = dr[i] & 0xffffu;
-            const uint32_t first = (cw[d >> 1] >> ((d & 1u) << 4)) & 0xffffu;
-            const uint32_t pos = (first + (dr[i] >> 16)) & (NS - 1);
+            const uint32_t pos = cw[d[i]] + r[i];
             ks[pos] = k[i];
             if (IDX) vs[pos] = v[i];
         }
         wave_sync();
     };
-    load();
-    U mn = key_max<U>(), mx = 0;
+    U mn, mx;
+    load(mn, mx);
 #pragma unroll
-    for (int i = 0; i < IT; i++) {
-        const uint32_t j = (uint32_t)i * WAVE + l;
-        if (j < arr_size) {
-            const U key = ks[j];
-            mn = key < mn ? key : mn;
-            mx = key > mx ? key : mx;
-        }
-    }
-#pragma unroll
-    for (int d = WAVE / 2; d >= 1; d >>= 1) {
-        const U omn = (U)__shfl_xor((uint32_t)mn, d, WAVE), omx = (U)__shfl_xor((uint32_t)mx, d, WAVE);
+    for (int dd = WAVE / 2; dd >= 1; dd >>= 1) {
+        const U omn = (U)__shfl_xor((uint32_t)mn, dd, WAVE), omx = (U)__shfl_xor((uint32_t)mx, dd, WAVE);
         mn = omn < mn ? omn : mn;
         mx = omx > mx ? omx : mx;
     }
-    const uint32_t span = (uint32_t)(U)(mx - mn);
+    mn = (U)__builtin_amdgcn_readfirstlane((int)(uint32_t)mn);
+    const uint32_t span = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(U)(mx - mn));
     const uint32_t bits = span ? 32u - (uint32_t)__clz(span) : 0u;
     const uint32_t nfull = (bits + RB - 1) / RB;
     bool top = nfull > 2;   // wave-uniform
@@ -360,22 +362,22 @@ __global__ void __launch_bounds__(256) ms_radix(const U *__restrict__ in, U *__r
     // offset from the minimum, clamped to the span: padding positions (the
     // largest key) tie with the array's maximum and, being later, stay behind
     auto rel = [&](U key) {
-        const uint32_t r = (uint32_t)(U)(key - mn);
-        return r < span ? r : span;
+        const uint32_t q = (uint32_t)(U)(key - mn);
+        return q < span ? q : span;
     };
     auto top_of = [&](U key) { return rel(key) >> lowb; };
 #pragma nounroll
     for (uint32_t attempt = 0; attempt < 2; attempt++) {   // wave-uniform
         if (attempt == 1) {
+            U m0, m1;
             wave_sync();
-            load();
+            load(m0, m1);
         }
         const uint32_t npass = top ? 2u : nfull;
 #pragma nounroll
         for (uint32_t p = 0; p < npass; p++) {
             const uint32_t sh = lowb + p * RB;
-            pass([&](U key) { return (rel(key) >> sh) & (NBIN - 1); },
-                 std::integral_constant<uint32_t, NBIN>());
+            pass([&](U key) { return (rel(key) >> sh) & (NBIN - 1); });
         }
         if (top) {
             // groups: runs of keys with equal top bits (position order
@@ -389,10 +391,11 @@ __global__ void __launch_bounds__(256) ms_radix(const U *__restrict__ in, U *__r
             uint32_t eq = 0, last = 0;
 #pragma unroll
             for (int i = 0; i < IT; i++) {
-                const uint32_t t = top_of(ks[(uint32_t)i * WAVE + l]);
-                const uint32_t before = (uint32_t)__builtin_amdgcn_update_dpp((int)last, (int)t, 0x138, 0xf, 0xf, false);
                 const uint32_t j = (uint32_t)i * WAVE + l;
-                if (j < arr_size && j > 0 && before == t) eq |= 1u << i;
+                const uint32_t t = top_of(ks[j]);
+                const uint32_t before = (uint32_t)__builtin_amdgcn_update_dpp((int)last, (int)t, 0x138, 0xf, 0xf, false);
+                const bool ok = (i > 0 || l > 0) && (i < IT_FULL || j < arr_size);
+                if (ok && before == t) eq |= 1u << i;
                 last = (uint32_t)__builtin_amdgcn_readlane((int)t, WAVE - 1);
             }
             // a group's head has no eq bit and the key after it has one (lane
@@ -440,16 +443,14 @@ __global__ void __launch_bounds__(256) ms_radix(const U *__restrict__ in, U *__r
             }
             wave_sync();
         }
-        // write-out in position order from LDS
+        // write-out in position order from LDS (stores past the end dropped)
         uint32_t l = lane();
         asm volatile("" : "+v"(l));
 #pragma unroll
         for (int i = 0; i < IT; i++) {
             const uint32_t j = (uint32_t)i * WAVE + l;
-            if (j < arr_size) {
-                out[base + j] = from_ord<U, FL>(ks[j], flip);
-                if (IDX) iout[base + j] = vs[j];
-            }
+            buf_store<U>(rout, j * (uint32_t)sizeof(U), from_ord<U, FL>(ks[j], flip));
+            if (IDX) buf_store<uint32_t>(riout, j * 4u, vs[j]);
         }
         break;
     }
