@@ -270,14 +270,12 @@ __global__ void __launch_bounds__(256) ms_radix(const U *__restrict__ in, U *__r
     U *const ks = sk[w];
     uint32_t *const vs = si[IDX ? w : 0];
     uint32_t *const cw = cnt[w];
-    // load: keys (order representation) and indices into LDS in position
-    // order; positions past the array's end hold the largest key.  Returns
-    // the lane's smallest and largest key.
-    auto load = [&](U &mn, U &mx) {
+    // load: keys (order representation) and indices of the lane's positions
+    // i*64 + l into registers; positions past the array's end hold the
+    // largest key.  Also returns the lane's smallest and largest key.
+    auto load = [&](U (&k)[IT], uint32_t (&v)[IT], U &mn, U &mx) {
         uint32_t l = lane();
         asm volatile("" : "+v"(l));   // keeps per-lane addresses out of the enclosing loops
-        U k[IT];
-        uint32_t v[IT];
 #pragma unroll
         for (int i = 0; i < IT; i++) {
             const uint32_t j = (uint32_t)i * WAVE + l;
@@ -293,23 +291,15 @@ __global__ void __launch_bounds__(256) ms_radix(const U *__restrict__ in, U *__r
             const U key = to_ord<U, FL>(k[i], flip);
             mn = ok && key < mn ? key : mn;
             mx = ok && key > mx ? key : mx;
-            ks[j] = ok ? key : key_max<U>();
-            if (IDX) vs[j] = v[i];
+            k[i] = ok ? key : key_max<U>();
         }
-        wave_sync();
     };
-    // one stable pass: digit(key) of the 16 items each lane reads in position
-    // order, ranks through the counters, 64-lane scan, placement in LDS
-    auto pass = [&](auto digit) {
+    // one stable pass over the keys held in registers (position order):
+    // digit(key) of every item, ranks through the counters, 64-lane scan,
+    // placement in LDS
+    auto pass_regs = [&](const U (&k)[IT], const uint32_t (&v)[IT], auto digit) {
         uint32_t l = lane();
         asm volatile("" : "+v"(l));
-        U k[IT];
-        uint32_t v[IT];
-#pragma unroll
-        for (int i = 0; i < IT; i++) {
-            k[i] = ks[(uint32_t)i * WAVE + l];
-            if (IDX) v[i] = vs[(uint32_t)i * WAVE + l];
-        }
 #pragma unroll
         for (uint32_t c = 0; c < CPL; c++) cw[l * CPL + c] = 0;
         wave_sync();
@@ -345,20 +335,22 @@ __global__ void __launch_bounds__(256) ms_radix(const U *__restrict__ in, U *__r
         }
         wave_sync();
     };
-    U mn, mx;
-    load(mn, mx);
+    // the same over the keys LDS holds in position order
+    auto pass_lds = [&](auto digit) {
+        uint32_t l = lane();
+        asm volatile("" : "+v"(l));
+        U k[IT];
+        uint32_t v[IT];
 #pragma unroll
-    for (int dd = WAVE / 2; dd >= 1; dd >>= 1) {
-        const U omn = (U)__shfl_xor((uint32_t)mn, dd, WAVE), omx = (U)__shfl_xor((uint32_t)mx, dd, WAVE);
-        mn = omn < mn ? omn : mn;
-        mx = omx > mx ? omx : mx;
-    }
-    mn = (U)__builtin_amdgcn_readfirstlane((int)(uint32_t)mn);
-    const uint32_t span = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(U)(mx - mn));
-    const uint32_t bits = span ? 32u - (uint32_t)__clz(span) : 0u;
-    const uint32_t nfull = (bits + RB - 1) / RB;
-    bool top = nfull > 2;   // wave-uniform
-    uint32_t lowb = top ? bits - TOPB : 0u;
+        for (int i = 0; i < IT; i++) {
+            k[i] = ks[(uint32_t)i * WAVE + l];
+            if (IDX) v[i] = vs[(uint32_t)i * WAVE + l];
+        }
+        pass_regs(k, v, digit);
+    };
+    U mn = 0;
+    uint32_t span = 0, nfull = 0, lowb = 0;
+    bool top = false;
     // offset from the minimum, clamped to the span: padding positions (the
     // largest key) tie with the array's maximum and, being later, stay behind
     auto rel = [&](U key) {
@@ -368,16 +360,42 @@ __global__ void __launch_bounds__(256) ms_radix(const U *__restrict__ in, U *__r
     auto top_of = [&](U key) { return rel(key) >> lowb; };
 #pragma nounroll
     for (uint32_t attempt = 0; attempt < 2; attempt++) {   // wave-uniform
-        if (attempt == 1) {
-            U m0, m1;
-            wave_sync();
-            load(m0, m1);
+        {
+            U k[IT], lmn, lmx;
+            uint32_t v[IT];
+            load(k, v, lmn, lmx);
+            if (attempt == 0) {
+#pragma unroll
+                for (int dd = WAVE / 2; dd >= 1; dd >>= 1) {
+                    const U omn = (U)__shfl_xor((uint32_t)lmn, dd, WAVE), omx = (U)__shfl_xor((uint32_t)lmx, dd, WAVE);
+                    lmn = omn < lmn ? omn : lmn;
+                    lmx = omx > lmx ? omx : lmx;
+                }
+                mn = (U)__builtin_amdgcn_readfirstlane((int)(uint32_t)lmn);
+                span = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(U)(lmx - mn));
+                const uint32_t bits = span ? 32u - (uint32_t)__clz(span) : 0u;
+                nfull = (bits + RB - 1) / RB;
+                top = nfull > 2;
+                lowb = top ? bits - TOPB : 0u;
+            }
+            if (nfull == 0) {   // every key equal: stage them for the write-out
+                uint32_t l = lane();
+                asm volatile("" : "+v"(l));
+#pragma unroll
+                for (int i = 0; i < IT; i++) {
+                    ks[(uint32_t)i * WAVE + l] = k[i];
+                    if (IDX) vs[(uint32_t)i * WAVE + l] = v[i];
+                }
+                wave_sync();
+            } else {
+                pass_regs(k, v, [&](U key) { return (rel(key) >> lowb) & (NBIN - 1); });
+            }
         }
         const uint32_t npass = top ? 2u : nfull;
 #pragma nounroll
-        for (uint32_t p = 0; p < npass; p++) {
+        for (uint32_t p = 1; p < npass; p++) {
             const uint32_t sh = lowb + p * RB;
-            pass([&](U key) { return (rel(key) >> sh) & (NBIN - 1); });
+            pass_lds([&](U key) { return (rel(key) >> sh) & (NBIN - 1); });
         }
         if (top) {
             // groups: runs of keys with equal top bits (position order
