@@ -1457,64 +1457,62 @@ struct LevelK {
                 block_sync();
             }
             if (KB != 8 || !top) break;
-            // groups: a key whose neighbour (task order) has the same top
-            // bits takes its rank inside the group by comparisons (ties by
-            // position, so the order stays stable)
-            const U *tk = sk + wb * WS;   // the task's keys in their sorted slots
-            uint32_t pos[SI];
-            bool grp[SI], ovf = false, any = false;
-            if (act) {
+            // groups: runs of keys with equal top bits (neighbours after the
+            // passes).  The lane holding a group's first position (its head:
+            // the key before has other top bits, the key after the same)
+            // orders the group in LDS by insertion sort (stable).  Top bits
+            // are below 2^TOPB, so they are compared as 32-bit values; the
+            // passes only permute a group's keys among its positions, so the
+            // top bits seen at any position do not change during the fix-up.
+            U *const tk = sk + wb * WS;   // the task's keys in their sorted slots
+            uint32_t *const ti = si + wb * WS;
+            auto top32 = [&](U key) { return (uint32_t)((U)(key - tmin) >> lowb); };
+            uint32_t heads = 0;
+            if (act && lowb != 0) {   // wave-uniform
 #pragma unroll
                 for (int i = 0; i < SI; i++) {
                     const uint32_t j = j0 + (uint32_t)i * WAVE;
-                    k[i] = rk[i * WAVE];
-                    if (lowb == 0) {   // wave-uniform: the task is sorted
-                        grp[i] = false;
-                        pos[i] = j;
-                        continue;
-                    }
-                    const U pf = (U)((U)(k[i] - tmin) >> lowb);
-                    const U pl = j > 0 && j <= len ? (U)((U)(tk[j - 1] - tmin) >> lowb) : (U)~pf;
-                    const U pr = j + 1 < len ? (U)((U)(tk[j + 1] - tmin) >> lowb) : (U)~pf;
-                    grp[i] = j < len && (pl == pf || pr == pf);
-                    pos[i] = j;
-                    any |= grp[i];
-                }
-                if (ballot(any)) {   // wave-uniform: rare for spread keys
-#pragma unroll
-                    for (int i = 0; i < SI; i++) {
-                        if (grp[i]) {
-                            const uint32_t j = j0 + (uint32_t)i * WAVE;
-                            const U pf = (U)((U)(k[i] - tmin) >> lowb);
-                            uint32_t gs = j, ge = j + 1;
-                            while (gs > 0 && j - gs < GMAX && (U)((U)(tk[gs - 1] - tmin) >> lowb) == pf) gs--;
-                            while (ge < len && ge - j < GMAX && (U)((U)(tk[ge] - tmin) >> lowb) == pf) ge++;
-                            ovf |= j - gs >= GMAX || ge - j >= GMAX;
-                            uint32_t r = 0;
-                            for (uint32_t t = gs; t < ge; t++) {
-                                const U kt = tk[t];
-                                r += (kt < k[i] || (kt == k[i] && t < j)) ? 1u : 0u;
-                            }
-                            pos[i] = gs + r;
-                        }
-                    }
+                    const uint32_t t = top32(rk[i * WAVE]);
+                    const bool prev_eq = j > 0 && j < len && top32(tk[j - 1]) == t;
+                    const bool next_eq = j + 1 < len && top32(tk[j + 1]) == t;
+                    heads |= (!prev_eq && next_eq) ? (1u << i) : 0u;
                 }
             }
             if (tid == 0) rs.flag = 0;
             block_sync();
-            if (ovf) rs.flag = 1;   // every writer stores the same value
-            block_sync();
-            if (__builtin_amdgcn_readfirstlane(rs.flag) == 0) {
-                if (act) {
-#pragma unroll
-                    for (int i = 0; i < SI; i++) {
-                        buf_store<U>(kw, pos[i] * (uint32_t)KB, from_ord<U, FL>(k[i], flip));
-                        if (IDX) buf_store<uint32_t>(iw, pos[i] * 4u, ri[i * WAVE]);
+            bool ovf = false;
+            if (ballot(heads != 0)) {   // wave-uniform: a few groups per task
+#pragma nounroll
+                for (uint32_t g = heads; g; g &= g - 1) {
+                    const uint32_t gs = j0 + (uint32_t)__builtin_ctz(g) * WAVE;
+                    const uint32_t t = top32(tk[gs]);
+                    uint32_t ge = gs + 1;
+#pragma nounroll
+                    while (ge < len && ge - gs <= GMAX && top32(tk[ge]) == t) ge++;
+                    if (ge - gs > GMAX) {
+                        ovf = true;
+                        break;
+                    }
+#pragma nounroll
+                    for (uint32_t q = gs + 1; q < ge; q++) {
+                        const U kq = tk[q];
+                        const uint32_t vq = IDX ? ti[q] : 0u;
+                        uint32_t r = q;
+#pragma nounroll
+                        while (r > gs && tk[r - 1] > kq) {
+                            tk[r] = tk[r - 1];
+                            if (IDX) ti[r] = ti[r - 1];
+                            r--;
+                        }
+                        tk[r] = kq;
+                        if (IDX) ti[r] = vq;
                     }
                 }
-                block_sync();   // the next step rewrites the LDS arrays
-                return;
             }
+            if (ovf) rs.flag = 1;   // every writer stores the same value
+            block_sync();
+            if (__builtin_amdgcn_readfirstlane(rs.flag) == 0) break;   // sorted in LDS: the write-out below
+            // a group longer than GMAX: the full passes, from the input again
             top = false;
             lowb = 0;
             npass = nfull;
