@@ -1337,25 +1337,28 @@ struct LevelK {
         // task) order are NBIN per wave in use, so thread tid owns the EPT
         // consecutive entries from tid*EPT, all inside its own wave's task:
         // entry e1 + k2 of the task (e1 = (w - wb)*NBIN + l*EPT) is digit
-        // (e1 + k2) / kq, wave wb + (e1 + k2) % kq; kept as nibbles from
-        // (d0, w0).  Threads of idle waves own none.
-        static_assert(NBIN == (uint32_t)NT && EPT <= 8 && NW <= 8 && NBIN % EPT == 0, "scan ownership");
+        // (e1 + k2) / kq, wave wb + (e1 + k2) % kq; their counters' element
+        // offsets in rs.cnt (< NW*NBIN = 2^12) are kept two to a word.
+        // Threads of idle waves own none.
+        static_assert(NBIN == (uint32_t)NT && EPT <= 8 && EPT % 2 == 0 && NW * NBIN <= 65536 && NBIN % EPT == 0,
+                      "scan ownership");
         const uint32_t kq = rw.kq;
         const uint32_t e1 = (w - wb) * NBIN + l * EPT;
-        const uint32_t d0 = e1 / kq, w0 = e1 - d0 * kq;
-        uint32_t ej = 0, ew = 0;
+        uint32_t ea[EPT / 2];
         {
-            uint32_t jj = 0, ww = w0;
+            uint32_t jj = e1 / kq, ww = e1 - jj * kq;
 #pragma unroll
             for (int k2 = 0; k2 < (int)EPT; k2++) {
-                ej |= jj << (4 * k2);
-                ew |= ww << (4 * k2);
+                const uint32_t o = (wb + ww) * NBIN + jj;
+                if (k2 & 1) ea[k2 / 2] |= o << 16;
+                else ea[k2 / 2] = o;
                 if (++ww == kq) {
                     ww = 0;
                     jj++;
                 }
             }
         }
+        auto eoff = [&](int k2) { return (k2 & 1) ? ea[k2 / 2] >> 16 : ea[k2 / 2] & 0xffffu; };
         U *dst = (U *)p.kout;
         const BufRsrc kw = buf_rsrc(dst + rw.tk.begin, len * (uint32_t)KB);
         const BufRsrc iw = buf_rsrc(p.iout + rw.tk.begin, IDX ? len * 4u : 0u);
@@ -1432,7 +1435,7 @@ struct LevelK {
                     for (int k2 = 0; k2 < (int)EPT; k2++) {
                         cv[k2] = 0u;
                         if (act)   // wave-uniform
-                            cv[k2] = rs.cnt[wb + ((ew >> (4 * k2)) & 15u)][d0 + ((ej >> (4 * k2)) & 15u)];
+                            cv[k2] = (&rs.cnt[0][0])[eoff(k2)];
                         loc += cv[k2];
                     }
                     uint32_t tot;
@@ -1440,7 +1443,7 @@ struct LevelK {
                     if (act) {
 #pragma unroll
                         for (int k2 = 0; k2 < (int)EPT; k2++) {
-                            rs.cnt[wb + ((ew >> (4 * k2)) & 15u)][d0 + ((ej >> (4 * k2)) & 15u)] = (uint16_t)run;
+                            (&rs.cnt[0][0])[eoff(k2)] = (uint16_t)run;
                             run += cv[k2];
                         }
                     }
