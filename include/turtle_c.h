@@ -152,7 +152,7 @@ int turtle_sort_device(turtle_handle h, const void *keys_in, const int32_t *idx_
  * for callers that cannot allocate device memory: tight serving loops and
  * harnesses that forbid hipMalloc).  No handle: every buffer of the run
  * beyond the caller's inputs and outputs lives in a static device array of
- * the library (42.4 MiB, held on each GPU it runs on until the process
+ * the library (43.6 MiB, held on each GPU it runs on until the process
  * ends), so at most 1,048,576 keys (TURTLE_ERR_CAPACITY above that).  Calls
  * no hipMalloc, hipFree or other allocating routine.  dtype is a
  * turtle_dtype; idx_in/idx_out are both NULL or both set; keys_out may

@@ -1114,10 +1114,11 @@ struct LevelK {
     static constexpr uint32_t WS = C::WS;
     static constexpr uint32_t CRAD = (uint32_t)C::CRAD, CLAST = (uint32_t)C::CLAST;
 
-    // 8-byte keys sort every task with the merge path: after the partition
-    // their tasks still span ~50 bits for full-range keys, i.e. six 64-bit
-    // radix passes, which cost more than the log2(n/SI) merge rounds (the
-    // batched 20,000-segment int64 run: 12.7 -> 10.3 ms in the LDS sorts)
+    // MERGE_ONLY: every task takes the merge path.  Off: 8-byte keys, whose
+    // full-range leaves still span ~50 bits, take the radix with two passes
+    // over the top bits of each task's span and a fix-up of the keys that
+    // share them (radix_step), which beats the log2(n/SI) merge rounds (the
+    // batched 20,000-segment int64 run: 8.9 -> 5.9 ms in the LDS sorts).
     static constexpr bool MERGE_ONLY = false;
     static constexpr uint32_t CPLAN = MERGE_ONLY ? (uint32_t)C::CMAX + 1 : (uint32_t)CRAD;   // plan's first radix class
     static __device__ __forceinline__ uint32_t lds_class(uint32_t len)
