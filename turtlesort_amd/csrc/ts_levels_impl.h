@@ -1473,14 +1473,30 @@ struct LevelK {
             auto top32 = [&](U key) { return (uint32_t)((U)(key - tmin) >> lowb); };
             uint32_t heads = 0;
             if (act && lowb != 0) {   // wave-uniform
+                // eq bit i: the position before item i's (lane l-1 of item i,
+                // lane 63 of item i-1, or the previous wave's last position)
+                // has the same top bits.  The neighbour comes from a DPP
+                // whole-wave shift by one lane; lane 0 takes the value of the
+                // position before, read once for the wave.
+                const uint32_t wfirst = (w - wb) * WS, wnext = wfirst + WS;   // the wave's first position, the next wave's
+                uint32_t last = w > wb ? top32(tk[wfirst - 1]) : 0u;   // unused at position 0 (j > 0)
+                uint32_t eq = 0;
 #pragma unroll
                 for (int i = 0; i < SI; i++) {
                     const uint32_t j = j0 + (uint32_t)i * WAVE;
                     const uint32_t t = top32(rk[i * WAVE]);
-                    const bool prev_eq = j > 0 && j < len && top32(tk[j - 1]) == t;
-                    const bool next_eq = j + 1 < len && top32(tk[j + 1]) == t;
-                    heads |= (!prev_eq && next_eq) ? (1u << i) : 0u;
+                    const uint32_t before =
+                        (uint32_t)__builtin_amdgcn_update_dpp((int)last, (int)t, 0x138, 0xf, 0xf, false);
+                    if (j > 0 && j < len && before == t) eq |= 1u << i;
+                    last = (uint32_t)__builtin_amdgcn_readlane((int)t, WAVE - 1);
                 }
+                // the position after: lane l+1 of item i, lane 0 of item i+1,
+                // or for the wave's last position the next wave's first
+                const uint32_t eq_first = (uint32_t)__builtin_amdgcn_readfirstlane((int)eq);
+                const bool next_wave_eq = wnext < len && top32(tk[wnext]) == last;   // last: the wave's last position
+                const uint32_t old = (eq_first >> 1) | (next_wave_eq ? 1u << (SI - 1) : 0u);
+                const uint32_t after = (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)eq, 0x130, 0xf, 0xf, false);
+                heads = after & ~eq;
             }
             if (tid == 0) rs.flag = 0;
             block_sync();
