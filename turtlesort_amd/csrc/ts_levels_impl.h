@@ -1146,7 +1146,7 @@ struct LevelK {
         uint32_t coff[LDS_CLASSES];       // LDS-sort plan (lds_plan)
         uint32_t cwg[LDS_CLASSES + 1];
         uint32_t mix;                     // mixed 3+3+2-wave steps (lds_plan)
-        alignas(4) uint16_t cnt[NW][NBIN];   // per-wave digit counts (rank16_add), then the digits' first slots
+        alignas(16) uint16_t cnt[NW][NBIN];   // per-wave digit counts (rank16_add), then the digits' first slots
         U wmin[NW], wmax[NW];             // per-wave key extremes of the step
         uint32_t red[16];
         uint32_t flag;                    // a group of the step exceeded the group limit
@@ -1389,7 +1389,10 @@ struct LevelK {
                 }
                 uint32_t dr[SI];
                 if (act) {
-                    for (uint32_t d = l; d < NBIN; d += WAVE) rs.cnt[w][d] = 0;
+                    // the wave's 512 counters: 16 bytes per lane, one store each
+                    static_assert(NBIN * sizeof(uint16_t) == WAVE * 16, "one 16-byte store per lane clears a row");
+                    typedef uint32_t U32x4 __attribute__((ext_vector_type(4)));
+                    reinterpret_cast<U32x4 *>(&rs.cnt[w][0])[l] = U32x4{0u, 0u, 0u, 0u};
                     wave_sync();
                     // after a pass the task's keys are at its first len positions
                     auto digit = [&](int i) {
@@ -1431,21 +1434,74 @@ struct LevelK {
                 // exclusive scan over (task, digit, wave of the task): the first
                 // slot of every (wave, digit) run of the step
                 {
+                    // For a task of 1, 2 or 4 waves the thread's entries are
+                    // EPT/kq consecutive digits (aligned) in each of the
+                    // task's kq counter rows: one vector load and store per
+                    // row; other widths go entry by entry.
+                    static_assert(EPT == 8, "vector forms of the scan entries");
+                    typedef uint16_t H8 __attribute__((ext_vector_type(8)));
+                    typedef uint16_t H4 __attribute__((ext_vector_type(4)));
+                    typedef uint16_t H2 __attribute__((ext_vector_type(2)));
                     uint32_t cv[EPT], loc = 0;
 #pragma unroll
-                    for (int k2 = 0; k2 < (int)EPT; k2++) {
-                        cv[k2] = 0u;
-                        if (act)   // wave-uniform
-                            cv[k2] = (&rs.cnt[0][0])[eoff(k2)];
-                        loc += cv[k2];
+                    for (int k2 = 0; k2 < (int)EPT; k2++) cv[k2] = 0u;
+                    if (act) {   // wave-uniform, and so is kq
+                        if (kq == 1) {
+                            const H8 a = *reinterpret_cast<const H8 *>(&rs.cnt[wb][e1]);
+#pragma unroll
+                            for (int k2 = 0; k2 < 8; k2++) cv[k2] = a[k2];
+                        } else if (kq == 2) {
+                            const H4 a = *reinterpret_cast<const H4 *>(&rs.cnt[wb][e1 / 2]);
+                            const H4 b = *reinterpret_cast<const H4 *>(&rs.cnt[wb + 1][e1 / 2]);
+#pragma unroll
+                            for (int k2 = 0; k2 < 8; k2++) cv[k2] = (k2 & 1) ? b[k2 / 2] : a[k2 / 2];
+                        } else if (kq == 4) {
+                            H2 r[4];
+#pragma unroll
+                            for (int q = 0; q < 4; q++) r[q] = *reinterpret_cast<const H2 *>(&rs.cnt[wb + q][e1 / 4]);
+#pragma unroll
+                            for (int k2 = 0; k2 < 8; k2++) cv[k2] = r[k2 % 4][k2 / 4];
+                        } else {
+#pragma unroll
+                            for (int k2 = 0; k2 < (int)EPT; k2++) cv[k2] = (&rs.cnt[0][0])[eoff(k2)];
+                        }
                     }
+#pragma unroll
+                    for (int k2 = 0; k2 < (int)EPT; k2++) loc += cv[k2];
                     uint32_t tot;
                     uint32_t run = block_excl_scan<NT>(loc, rs.red, &tot);
                     if (act) {
+                        uint16_t o[EPT];   // the entries' exclusive prefixes, in entry order
 #pragma unroll
                         for (int k2 = 0; k2 < (int)EPT; k2++) {
-                            (&rs.cnt[0][0])[eoff(k2)] = (uint16_t)run;
+                            o[k2] = (uint16_t)run;
                             run += cv[k2];
+                        }
+                        if (kq == 1) {
+                            H8 a;
+#pragma unroll
+                            for (int k2 = 0; k2 < 8; k2++) a[k2] = o[k2];
+                            *reinterpret_cast<H8 *>(&rs.cnt[wb][e1]) = a;
+                        } else if (kq == 2) {
+                            H4 a, b;
+#pragma unroll
+                            for (int k2 = 0; k2 < 4; k2++) {
+                                a[k2] = o[2 * k2];
+                                b[k2] = o[2 * k2 + 1];
+                            }
+                            *reinterpret_cast<H4 *>(&rs.cnt[wb][e1 / 2]) = a;
+                            *reinterpret_cast<H4 *>(&rs.cnt[wb + 1][e1 / 2]) = b;
+                        } else if (kq == 4) {
+#pragma unroll
+                            for (int q = 0; q < 4; q++) {
+                                H2 r;
+                                r[0] = o[q];
+                                r[1] = o[q + 4];
+                                *reinterpret_cast<H2 *>(&rs.cnt[wb + q][e1 / 4]) = r;
+                            }
+                        } else {
+#pragma unroll
+                            for (int k2 = 0; k2 < (int)EPT; k2++) (&rs.cnt[0][0])[eoff(k2)] = o[k2];
                         }
                     }
                 }
