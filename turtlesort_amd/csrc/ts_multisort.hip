@@ -404,13 +404,20 @@ __global__ void __launch_bounds__(256) ms_radix(const U *__restrict__ in, U *__r
             // join).  eq bit i: the key before item i's has the same top bits.
             // The neighbour comes from a DPP whole-wave shift by one lane,
             // lane 0 taking lane 63 of the item before.
+            // The keys (and indices) read here are written out from these
+            // registers, except for items where some lane holds a group
+            // member (the fix-up below may have moved those).
             uint32_t l = lane();
             asm volatile("" : "+v"(l));
+            U kr[IT];
+            uint32_t vr[IDX ? IT : 1];
             uint32_t eq = 0, last = 0;
 #pragma unroll
             for (int i = 0; i < IT; i++) {
                 const uint32_t j = (uint32_t)i * WAVE + l;
-                const uint32_t t = top_of(ks[j]);
+                kr[i] = ks[j];
+                if (IDX) vr[i] = vs[j];
+                const uint32_t t = top_of(kr[i]);
                 const uint32_t before = (uint32_t)__builtin_amdgcn_update_dpp((int)last, (int)t, 0x138, 0xf, 0xf, false);
                 const bool ok = (i > 0 || l > 0) && (i < IT_FULL || j < arr_size);
                 if (ok && before == t) eq |= 1u << i;
@@ -460,8 +467,23 @@ __global__ void __launch_bounds__(256) ms_radix(const U *__restrict__ in, U *__r
                 continue;
             }
             wave_sync();
+            const uint32_t grp = eq | after;   // this lane's items that are group members
+#pragma unroll
+            for (int i = 0; i < IT; i++) {
+                const uint32_t j = (uint32_t)i * WAVE + l;
+                U key = kr[i];
+                uint32_t val = IDX ? vr[IDX ? i : 0] : 0u;
+                if (ballot((grp >> i) & 1u)) {   // wave-uniform
+                    key = ks[j];
+                    if (IDX) val = vs[j];
+                }
+                buf_store<U>(rout, j * (uint32_t)sizeof(U), from_ord<U, FL>(key, flip));
+                if (IDX) buf_store<uint32_t>(riout, j * 4u, val);
+            }
+            break;
         }
-        // write-out in position order from LDS (stores past the end dropped)
+        // exact passes: write-out in position order from LDS (stores past the
+        // end dropped)
         uint32_t l = lane();
         asm volatile("" : "+v"(l));
 #pragma unroll
