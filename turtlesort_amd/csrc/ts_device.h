@@ -83,14 +83,27 @@ __device__ __forceinline__ uint32_t mask_rank(uint64_t m)
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+// Inclusive prefix sum over the 64 lanes (every lane active) from DPP lane
+// moves, no LDS traffic: shifts by 1, 2 and 3 inside each row of 16 lanes
+// give sums of four, shifts by 4 and 8 (upper banks only) row prefixes, and
+// the row broadcasts of lanes 15 and 31 carry the row totals into the rows
+// above.  Lanes a move has no source for, or that its row and bank masks
+// leave out, receive 0.
+template <int CTRL, int ROW_MASK, int BANK_MASK>
+__device__ __forceinline__ uint32_t dpp_or0(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, BANK_MASK, false);
+}
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
 {
-#pragma unroll
-    for (int d = 1; d < WAVE; d <<= 1) {
-        uint32_t o = __shfl_up(v, d, WAVE);
-        if ((int)lane() >= d) v += o;
-    }
-    return v;
+    uint32_t s = v + dpp_or0<0x111, 0xf, 0xf>(v);   // row_shr:1
+    s += dpp_or0<0x112, 0xf, 0xf>(v);               // row_shr:2
+    s += dpp_or0<0x113, 0xf, 0xf>(v);               // row_shr:3
+    s += dpp_or0<0x114, 0xf, 0xe>(s);               // row_shr:4, lanes 4..15 of each row
+    s += dpp_or0<0x118, 0xf, 0xc>(s);               // row_shr:8, lanes 8..15 of each row
+    s += dpp_or0<0x142, 0xa, 0xf>(s);               // row_bcast:15 into rows 1 and 3
+    s += dpp_or0<0x143, 0xc, 0xf>(s);               // row_bcast:31 into rows 2 and 3
+    return s;
 }
 
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
