@@ -106,6 +106,43 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
     return s;
 }
 
+// Wave-wide minimum or maximum (every lane active; every lane gets it): the
+// DPP moves of wave_incl_scan with the operation's identity in the lanes a
+// move leaves out, then lane 63's running result.  T: 32- or 64-bit.
+template <int CTRL, int ROW_MASK, int BANK_MASK, typename T>
+__device__ __forceinline__ T dpp_or(T ident, T v)
+{
+    if constexpr (sizeof(T) == 8) {
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp((int)(uint32_t)ident, (int)(uint32_t)v, CTRL, ROW_MASK,
+                                                                  BANK_MASK, false);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp((int)(uint32_t)(ident >> 32), (int)(uint32_t)(v >> 32),
+                                                                  CTRL, ROW_MASK, BANK_MASK, false);
+        return (T)(((uint64_t)hi << 32) | lo);
+    } else {
+        return (T)(uint32_t)__builtin_amdgcn_update_dpp((int)(uint32_t)ident, (int)(uint32_t)v, CTRL, ROW_MASK,
+                                                        BANK_MASK, false);
+    }
+}
+template <bool MAX, typename T>
+__device__ __forceinline__ T wave_extreme(T v)
+{
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32- or 64-bit values");
+    const T ident = MAX ? (T)0 : (T)~(T)0;
+    auto op = [](T a, T b) { return MAX ? (a > b ? a : b) : (a < b ? a : b); };
+    T s = op(v, dpp_or<0x111, 0xf, 0xf>(ident, v));
+    s = op(s, dpp_or<0x112, 0xf, 0xf>(ident, v));
+    s = op(s, dpp_or<0x113, 0xf, 0xf>(ident, v));
+    s = op(s, dpp_or<0x114, 0xf, 0xe>(ident, s));
+    s = op(s, dpp_or<0x118, 0xf, 0xc>(ident, s));
+    s = op(s, dpp_or<0x142, 0xa, 0xf>(ident, s));
+    s = op(s, dpp_or<0x143, 0xc, 0xf>(ident, s));
+    if constexpr (sizeof(T) == 8)
+        return (T)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(s >> 32), WAVE - 1) << 32) |
+                   (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)s, WAVE - 1));
+    else
+        return (T)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)s, WAVE - 1);
+}
+
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
 {
 #pragma unroll

@@ -1168,24 +1168,10 @@ struct LevelK {
             return (T)__shfl((uint32_t)x, src, WAVE);
     }
 
-    static __device__ __forceinline__ U wave_min(U x)
-    {
-#pragma unroll
-        for (int d = WAVE / 2; d >= 1; d >>= 1) {
-            const U o = (U)__shfl_xor(x, d, WAVE);
-            x = o < x ? o : x;
-        }
-        return x;
-    }
-    static __device__ __forceinline__ U wave_max(U x)
-    {
-#pragma unroll
-        for (int d = WAVE / 2; d >= 1; d >>= 1) {
-            const U o = (U)__shfl_xor(x, d, WAVE);
-            x = o > x ? o : x;
-        }
-        return x;
-    }
+    // (narrow keys widened to 32 bits; the reductions run on DPP moves)
+    typedef typename std::conditional<KB == 8, uint64_t, uint32_t>::type UW;
+    static __device__ __forceinline__ U wave_min(U x) { return (U)wave_extreme<false, UW>((UW)x); }
+    static __device__ __forceinline__ U wave_max(U x) { return (U)wave_extreme<true, UW>((UW)x); }
 
     // The radix classes' workgroup steps.  Step v of class c (kw = c - CRAD
     // + 1 waves per task, P = NW / kw tasks) holds the class's tasks

@@ -365,12 +365,8 @@ __global__ void __launch_bounds__(256) ms_radix(const U *__restrict__ in, U *__r
             uint32_t v[IT];
             load(k, v, lmn, lmx);
             if (attempt == 0) {
-#pragma unroll
-                for (int dd = WAVE / 2; dd >= 1; dd >>= 1) {
-                    const U omn = (U)__shfl_xor((uint32_t)lmn, dd, WAVE), omx = (U)__shfl_xor((uint32_t)lmx, dd, WAVE);
-                    lmn = omn < lmn ? omn : lmn;
-                    lmx = omx > lmx ? omx : lmx;
-                }
+                lmn = (U)wave_extreme<false, uint32_t>((uint32_t)lmn);   // DPP reductions, no LDS traffic
+                lmx = (U)wave_extreme<true, uint32_t>((uint32_t)lmx);
                 mn = (U)__builtin_amdgcn_readfirstlane((int)(uint32_t)lmn);
                 span = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(U)(lmx - mn));
                 const uint32_t bits = span ? 32u - (uint32_t)__clz(span) : 0u;
