@@ -1159,13 +1159,16 @@ struct LevelK {
         uint32_t tbase, done;         // the step's first tile; segments done
     };
 
+    // lane src's x for every lane; src is the same for every lane (a scalar
+    // read of the lane, no LDS permute)
     template <typename T>
     static __device__ __forceinline__ T shfl_u(T x, int src)
     {
         if constexpr (sizeof(T) == 8)
-            return (T)__shfl((unsigned long long)x, src, WAVE);
+            return (T)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)x >> 32), src) << 32) |
+                       (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(uint64_t)x, src));
         else
-            return (T)__shfl((uint32_t)x, src, WAVE);
+            return (T)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, src);
     }
 
     // (narrow keys widened to 32 bits; the reductions run on DPP moves)
