@@ -230,7 +230,9 @@ __device__ __forceinline__ void store_relaxed(uint32_t *p, uint32_t v)
 // red: LDS scratch of at least NT/64 + 1 words.  Returns the exclusive prefix
 // of v over thread order; *total receives the block sum.  Contains barriers:
 // every thread of the block must call it.
-template <int NT>
+// TRAILING: end with a barrier, so that red may be rewritten right away;
+// without it the caller passes a barrier before red is written again.
+template <int NT, bool TRAILING = true>
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *red, uint32_t *total)
 {
     constexpr int NW = NT / WAVE;
@@ -245,7 +247,7 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *red, u
         wpre += (i < w) ? s : 0u;
         tot += s;
     }
-    block_sync();
+    if (TRAILING) block_sync();
     *total = tot;
     return wpre + inc - v;
 }

@@ -1458,7 +1458,9 @@ struct LevelK {
 #pragma unroll
                     for (int k2 = 0; k2 < (int)EPT; k2++) loc += cv[k2];
                     uint32_t tot;
-                    uint32_t run = block_excl_scan<NT>(loc, rs.red, &tot);
+                    // (no trailing barrier: the one after the prefixes are
+                    // stored comes before rs.red is written again)
+                    uint32_t run = block_excl_scan<NT, false>(loc, rs.red, &tot);
                     if (act) {
                         uint16_t o[EPT];   // the entries' exclusive prefixes, in entry order
 #pragma unroll
@@ -1600,8 +1602,11 @@ struct LevelK {
                 if (IDX) buf_store<uint32_t>(iw, j * 4u, v[i]);
             }
         }
-        // the next step rewrites the LDS arrays
-        block_sync();
+        // (no barrier here: before the next step writes the key slots, the
+        // counters or the group flag, every wave has passed that step's
+        // range barrier, i.e. has finished this step's reads; the only LDS
+        // it writes before then are its own wmin/wmax entries, read here only
+        // before this step's first pass)
     }
 
     // all workgroup steps of the radix classes [CRAD, CLAST]
