@@ -1150,11 +1150,10 @@ static int multisort_device(turtle_handle h, int dtype, int64_t arr_size, const 
     if ((uint64_t)arr_size <= ts::multisort_max_size(h->kt)) {
         const int rc = resolve(h);   // the previous engine run's level check
         if (rc) return rc;
-        HIPCHK(hipEventRecord(h->ev0, st), "hipEventRecord");
+        // (the launches carry the timing events themselves)
         HIPCHK(ts::multisort_run(h->kt, h->flip, (uint32_t)arr_size, keys_in, keys_out, (const uint32_t *)idx_in,
-                                 (uint32_t *)idx_out, (uint64_t)num_arrays, st),
+                                 (uint32_t *)idx_out, (uint64_t)num_arrays, st, h->ev0, h->ev1),
                "multisort launch");
-        HIPCHK(hipEventRecord(h->ev1, st), "hipEventRecord");
         h->timed = true;
         h->last_stream = st;
         h->last_n = (uint64_t)arr_size * num_arrays;

@@ -34,9 +34,11 @@ hipError_t engine_clear_classes(const Params &p, hipStream_t st);
 // kt: key type code.  iout != nullptr: a 32-bit payload moves with every
 // key, read from iin, or each key's position in its array when iin is
 // nullptr.  Returns hipErrorInvalidValue when arr_size exceeds
-// multisort_max_size(kt).
+// multisort_max_size(kt).  e0, e1 (or nullptr): events recorded at the
+// start and the end of the launches, carried by the launches themselves.
 hipError_t multisort_run(int kt, uint64_t flip, uint32_t arr_size, const void *in, void *out, const uint32_t *iin,
-                         uint32_t *iout, uint64_t num_arrays, hipStream_t st);
+                         uint32_t *iout, uint64_t num_arrays, hipStream_t st, hipEvent_t e0 = nullptr,
+                         hipEvent_t e1 = nullptr);
 uint32_t multisort_max_size(int kb);
 
 // Stable merge of two sorted runs (ts_merge.hip): out[0, na+nb) = a merged

@@ -11,6 +11,8 @@
 // (turtle-multi-sort.cuh:152 sorts an unfilled scratch buffer).
 #include <type_traits>
 
+#include <hip/hip_ext.h>
+
 #include "ts_device.h"
 #include "ts_internal.h"
 
@@ -492,20 +494,23 @@ __global__ void __launch_bounds__(256) ms_radix(const U *__restrict__ in, U *__r
     }
 }
 
+// Launches carry the caller's timing events (start, stop; either may be
+// null) in the kernel's own dispatch: timing a launch with separate event
+// records costs several microseconds of queue time per launch.
 template <typename U, bool FL, bool IDX, int N>
 static void small_launch(uint32_t arr_size, const U *in, U *out, const uint32_t *iin, uint32_t *iout, uint64_t na,
-                         U f, hipStream_t st)
+                         U f, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
     typedef typename std::conditional<sizeof(U) == 8, uint64_t, uint32_t>::type W;
     constexpr int NT = sizeof(U) == 8 ? 128 : 256;
     const uint64_t inv = ((1ull << 32) + arr_size - 1) / arr_size;
-    hipLaunchKernelGGL((ms_small<U, W, FL, IDX, N, NT>), dim3((uint32_t)((na + NT - 1) / NT)), dim3(NT), 0, st, in, out,
-                       iin, iout, arr_size, inv, na, f);
+    hipExtLaunchKernelGGL((ms_small<U, W, FL, IDX, N, NT>), dim3((uint32_t)((na + NT - 1) / NT)), dim3(NT), 0, st, e0,
+                          e1, 0, in, out, iin, iout, arr_size, inv, na, f);
 }
 
 template <typename U, bool FL, bool IDX>
 static hipError_t ms_launch(uint64_t flip, uint32_t arr_size, const void *in, void *out, const uint32_t *iin,
-                            uint32_t *iout, uint64_t num_arrays, hipStream_t st)
+                            uint32_t *iout, uint64_t num_arrays, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1)
 {
     const U f = (U)flip;
     const U *i = (const U *)in;
@@ -520,40 +525,42 @@ static hipError_t ms_launch(uint64_t flip, uint32_t arr_size, const void *in, vo
         U *oo = o + e0;
         const uint32_t *xi = iin ? iin + e0 : nullptr;
         uint32_t *xo = IDX ? iout + e0 : nullptr;
+        // the first chunk's launch starts the timing, the last one's stops it
+        hipEvent_t t0 = a0 == 0 ? ev0 : nullptr, t1 = a0 + na == num_arrays ? ev1 : nullptr;
         if (ne <= 32) {
             // one lane per array, a register network of ne wires
             switch (ne < 2 ? 2 : ne) {
-            case 2: small_launch<U, FL, IDX, 2>(arr_size, ii, oo, xi, xo, na, f, st); break;
-            case 4: small_launch<U, FL, IDX, 4>(arr_size, ii, oo, xi, xo, na, f, st); break;
-            case 8: small_launch<U, FL, IDX, 8>(arr_size, ii, oo, xi, xo, na, f, st); break;
-            case 16: small_launch<U, FL, IDX, 16>(arr_size, ii, oo, xi, xo, na, f, st); break;
-            default: small_launch<U, FL, IDX, 32>(arr_size, ii, oo, xi, xo, na, f, st); break;
+            case 2: small_launch<U, FL, IDX, 2>(arr_size, ii, oo, xi, xo, na, f, st, t0, t1); break;
+            case 4: small_launch<U, FL, IDX, 4>(arr_size, ii, oo, xi, xo, na, f, st, t0, t1); break;
+            case 8: small_launch<U, FL, IDX, 8>(arr_size, ii, oo, xi, xo, na, f, st, t0, t1); break;
+            case 16: small_launch<U, FL, IDX, 16>(arr_size, ii, oo, xi, xo, na, f, st, t0, t1); break;
+            default: small_launch<U, FL, IDX, 32>(arr_size, ii, oo, xi, xo, na, f, st, t0, t1); break;
             }
         } else if (ne == 1024 && sizeof(U) <= 4) {
             // 513..1024 keys of up to 4 bytes: LDS radix, one wave per array
             if constexpr (sizeof(U) <= 4)
-                hipLaunchKernelGGL((ms_radix<U, FL, IDX>), dim3((uint32_t)((na + 3) / 4)), dim3(256), 0, st, ii, oo,
-                                   xi, xo, arr_size, na, f);
+                hipExtLaunchKernelGGL((ms_radix<U, FL, IDX>), dim3((uint32_t)((na + 3) / 4)), dim3(256), 0, st, t0, t1,
+                                      0, ii, oo, xi, xo, arr_size, na, f);
         } else if (ne <= 1024) {
             // one wave of 16 keys per lane, 1024 / ne arrays per wave (no
             // cross-wave barriers; 16 keys per thread take one merge round
             // fewer than 8; 65,536 x 1024 int32 keys: 0.34 -> 0.29 ms)
             const uint32_t per = 1024u / ne;
             if (per > 1)
-                hipLaunchKernelGGL((ms_kernel<U, FL, IDX, 64, 16, true>), dim3((uint32_t)((na + per - 1) / per)),
-                                   dim3(64), 0, st, ii, oo, xi, xo, arr_size, per, na, f);
+                hipExtLaunchKernelGGL((ms_kernel<U, FL, IDX, 64, 16, true>), dim3((uint32_t)((na + per - 1) / per)),
+                                      dim3(64), 0, st, t0, t1, 0, ii, oo, xi, xo, arr_size, per, na, f);
             else
-                hipLaunchKernelGGL((ms_kernel<U, FL, IDX, 64, 16, false>), dim3((uint32_t)na), dim3(64), 0, st, ii, oo,
-                                   xi, xo, arr_size, 1u, na, f);
+                hipExtLaunchKernelGGL((ms_kernel<U, FL, IDX, 64, 16, false>), dim3((uint32_t)na), dim3(64), 0, st, t0,
+                                      t1, 0, ii, oo, xi, xo, arr_size, 1u, na, f);
         } else if (ne <= 2048)
-            hipLaunchKernelGGL((ms_kernel<U, FL, IDX, 128, 16, false>), dim3((uint32_t)na), dim3(128), 0, st, ii, oo,
-                               xi, xo, arr_size, 1u, na, f);
+            hipExtLaunchKernelGGL((ms_kernel<U, FL, IDX, 128, 16, false>), dim3((uint32_t)na), dim3(128), 0, st, t0,
+                                  t1, 0, ii, oo, xi, xo, arr_size, 1u, na, f);
         else if (ne <= 4096)
-            hipLaunchKernelGGL((ms_kernel<U, FL, IDX, 256, 16, false>), dim3((uint32_t)na), dim3(256), 0, st, ii, oo,
-                               xi, xo, arr_size, 1u, na, f);
+            hipExtLaunchKernelGGL((ms_kernel<U, FL, IDX, 256, 16, false>), dim3((uint32_t)na), dim3(256), 0, st, t0,
+                                  t1, 0, ii, oo, xi, xo, arr_size, 1u, na, f);
         else if (ne <= 8192)
-            hipLaunchKernelGGL((ms_kernel<U, FL, IDX, 512, 16, false>), dim3((uint32_t)na), dim3(512), 0, st, ii, oo,
-                               xi, xo, arr_size, 1u, na, f);
+            hipExtLaunchKernelGGL((ms_kernel<U, FL, IDX, 512, 16, false>), dim3((uint32_t)na), dim3(512), 0, st, t0,
+                                  t1, 0, ii, oo, xi, xo, arr_size, 1u, na, f);
         else
             return hipErrorInvalidValue;
         hipError_t e = hipGetLastError();
@@ -564,26 +571,26 @@ static hipError_t ms_launch(uint64_t flip, uint32_t arr_size, const void *in, vo
 
 template <typename U, bool FL>
 static hipError_t ms_pick(uint64_t flip, uint32_t arr_size, const void *in, void *out, const uint32_t *iin,
-                          uint32_t *iout, uint64_t num_arrays, hipStream_t st)
+                          uint32_t *iout, uint64_t num_arrays, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
-    return iout ? ms_launch<U, FL, true>(flip, arr_size, in, out, iin, iout, num_arrays, st)
-                : ms_launch<U, FL, false>(flip, arr_size, in, out, nullptr, nullptr, num_arrays, st);
+    return iout ? ms_launch<U, FL, true>(flip, arr_size, in, out, iin, iout, num_arrays, st, e0, e1)
+                : ms_launch<U, FL, false>(flip, arr_size, in, out, nullptr, nullptr, num_arrays, st, e0, e1);
 }
 
 uint32_t multisort_max_size(int) { return 8192; }
 
 hipError_t multisort_run(int kt, uint64_t flip, uint32_t arr_size, const void *in, void *out, const uint32_t *iin,
-                         uint32_t *iout, uint64_t num_arrays, hipStream_t st)
+                         uint32_t *iout, uint64_t num_arrays, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
     if (arr_size == 0 || num_arrays == 0) return hipSuccess;
     if (arr_size > multisort_max_size(kt)) return hipErrorInvalidValue;
     switch (kt) {
-    case 1: return ms_pick<uint8_t, false>(flip, arr_size, in, out, iin, iout, num_arrays, st);
-    case 2: return ms_pick<uint16_t, false>(flip, arr_size, in, out, iin, iout, num_arrays, st);
-    case 4: return ms_pick<uint32_t, false>(flip, arr_size, in, out, iin, iout, num_arrays, st);
-    case 8: return ms_pick<uint64_t, false>(flip, arr_size, in, out, iin, iout, num_arrays, st);
-    case 4 | KT_FLOAT: return ms_pick<uint32_t, true>(flip, arr_size, in, out, iin, iout, num_arrays, st);
-    case 8 | KT_FLOAT: return ms_pick<uint64_t, true>(flip, arr_size, in, out, iin, iout, num_arrays, st);
+    case 1: return ms_pick<uint8_t, false>(flip, arr_size, in, out, iin, iout, num_arrays, st, e0, e1);
+    case 2: return ms_pick<uint16_t, false>(flip, arr_size, in, out, iin, iout, num_arrays, st, e0, e1);
+    case 4: return ms_pick<uint32_t, false>(flip, arr_size, in, out, iin, iout, num_arrays, st, e0, e1);
+    case 8: return ms_pick<uint64_t, false>(flip, arr_size, in, out, iin, iout, num_arrays, st, e0, e1);
+    case 4 | KT_FLOAT: return ms_pick<uint32_t, true>(flip, arr_size, in, out, iin, iout, num_arrays, st, e0, e1);
+    case 8 | KT_FLOAT: return ms_pick<uint64_t, true>(flip, arr_size, in, out, iin, iout, num_arrays, st, e0, e1);
     default: return hipErrorInvalidValue;
     }
 }
