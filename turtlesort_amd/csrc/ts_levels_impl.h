@@ -1134,9 +1134,9 @@ struct LevelK {
     // Radix classes: a task's keys span a narrow range (they lie between two
     // splitters of their parent), so a stable LSD radix sort of key - min
     // over the range's bits takes a few passes of RW-bit digits, each pass
-    // ranking the keys with wave64 match masks and per-wave digit counters;
-    // padding slots take the largest digit in every pass and so stay behind
-    // the task's keys.
+    // ranking the keys with one LDS atomic per key on per-wave digit
+    // counters (rank16_add); padding slots take the largest digit in every
+    // pass and so stay behind the task's keys.
     static constexpr int RW = 9;
     static constexpr uint32_t NBIN = 1u << RW;
     static constexpr uint32_t MAXPASS = (8 * KB + RW - 1) / RW;   // covers any key range

@@ -228,23 +228,26 @@ __global__ void __launch_bounds__(NT) ms_small(const U *__restrict__ in, U *__re
 }
 
 // ---- arrays of 513..1024 keys of up to 4 bytes: one wave per array, LDS radix ------
-// Each wave sorts one array (four waves per workgroup, no block barrier):
-// its keys are staged in LDS in position order and sorted by stable LSD
-// passes over (key - min): per pass each lane reads 16 keys (positions
-// i*64 + l), a per-wave counter per 9-bit digit gives every key its rank
-// with one LDS atomic (rank16_add: the wave's keys of a digit in (item,
-// lane) order, i.e. in position order), a 64-lane scan of the counters
-// gives each digit's first slot, and the keys are placed in LDS.  Positions
-// past the array's end hold the largest key and stay behind its keys.
+// Each wave sorts one array (four waves per workgroup, no block barrier)
+// by stable LSD passes over (key - min), lane l holding positions i*64 + l
+// (16 items): the first pass ranks the keys straight from the registers
+// they were loaded into, later ones read them back from LDS in position
+// order.  Per pass a 32-bit per-wave counter per 9-bit digit gives every
+// key its rank with one LDS atomic (its return value: the wave's keys of a
+// digit in (item, lane) order, i.e. in position order; see rank16_add), a
+// 64-lane scan of the counters gives each digit's first slot, and the keys
+// are placed in LDS.  Positions past the array's end hold the largest key
+// and stay behind its keys.  The scans and the key range run on DPP moves.
 //
 // An array whose key span (max - min) is wider than two 9-bit digits is
 // sorted by its top 18 span bits only ((key - min) >> lowb, two passes);
 // keys that then agree in those bits (a group; for 1024 spread keys about
 // two pairs per array) are neighbours, found by comparing each key with
-// the one before it across lanes; the lane holding a group's first
-// position orders the group in LDS by insertion sort (stable).  A group
-// longer than GMAX makes the array take the exact passes instead (as does
-// a span of at most 18 bits): 9-bit digits over the whole span.
+// the one before it across lanes (DPP whole-wave shifts); the lane holding
+// a group's first position orders the group in LDS by insertion sort
+// (stable).  A group longer than GMAX makes the array take the exact passes
+// instead (as does a span of at most 18 bits): 9-bit digits over the whole
+// span.
 template <typename U, bool FL, bool IDX>
 __global__ void __launch_bounds__(256) ms_radix(const U *__restrict__ in, U *__restrict__ out,
                                                  const uint32_t *__restrict__ iin, uint32_t *__restrict__ iout,
