@@ -1779,7 +1779,7 @@ struct LevelK {
     // One wave per segment (8 per workgroup step): the wave samples 1024 keys
     // (128 evenly spaced runs of 8) into registers (16 per lane), sorts them
     // by stable LSD radix passes of RW-bit digits over the top 2*RW bits of
-    // the sample's key range (ranks from wave64 match masks and per-wave
+    // the sample's key range (ranks from one LDS atomic per key on per-wave
     // digit counters, no block barrier), takes S splitters at the sample
     // quantiles, orders them exactly and reserves the segment's tiles.
     static constexpr uint32_t SPL = C::SAMPLE / WAVE;   // sample keys per lane
