@@ -24,7 +24,8 @@ roofline   = the dominant kernel (largest summed device time in the
              profile summary.
 end_to_end = the same algorithmic bytes of all passes over the whole sort's
              device time (the "fraction of HBM roofline across the actual
-             number of passes")
+             number of passes"), from the mean of the per-sort device times
+             (HIP events around each timed sort; median and min beside it)
 cpu_baseline = the oracle's restatement of the reference algorithm (one
              core) on a bounded sample of the same distribution
 """
@@ -508,6 +509,8 @@ def main():
         "end_to_end": {
             "alg_bytes_per_sort": int(alg_bytes),
             "avg_sort_ms": round(avg_dev_ms, 4),
+            "median_sort_ms": round(sorted(dev_ms)[len(dev_ms) // 2], 4),
+            "min_sort_ms": round(min(dev_ms), 4),
             "achieved_GB_s": round(e2e, 1),
             "frac": round(e2e / HBM_PEAK_GBS, 4),
             "passes_equiv": round(alg_bytes / floor_bytes, 3),
