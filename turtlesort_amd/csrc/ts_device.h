@@ -287,6 +287,13 @@ __device__ __forceinline__ T buf_load(BufRsrc r, uint32_t off)
 // less dirty data in the L2s ends sooner (65,536 x 1024 int32 MultiSort:
 // 0.137 -> 0.127 ms per back-to-back launch).
 constexpr int CPOL_NT = 2;
+
+// the same for a plain pointer (a coalesced final-output store)
+template <typename T>
+__device__ __forceinline__ void nt_store(T *p, T v)
+{
+    __builtin_nontemporal_store(v, p);
+}
 template <typename T>
 __device__ __forceinline__ void buf_store(BufRsrc r, uint32_t off, T v)
 {

@@ -1757,8 +1757,8 @@ struct LevelK {
                     const uint32_t q = i >> c, j = i & (n_eff - 1);
                     const TaskRec tr = rec[q];
                     if (j < tr.len) {
-                        dst[tr.begin + j] = from_ord<U, FL>(mk[r * (NT + NT / SI)], flip);
-                        if (IDX) p.iout[tr.begin + j] = mi[r * (NT + NT / SI)];
+                        nt_store(&dst[tr.begin + j], from_ord<U, FL>(mk[r * (NT + NT / SI)], flip));
+                        if (IDX) nt_store(&p.iout[tr.begin + j], mi[r * (NT + NT / SI)]);
                     }
                 }
             }

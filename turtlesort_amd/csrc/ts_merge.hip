@@ -175,8 +175,8 @@ __global__ void __launch_bounds__(MERGE_NT) k_merge(const U *__restrict__ a, con
     for (int r = 0; r < IT; r++) {
         const uint32_t e = tid + (uint32_t)r * MERGE_NT;
         if (e < nt) {
-            out[d0 + e] = from_ord<U, FL>(sk[pidx<IT>(e)], flip);
-            if (IDX) iout[d0 + e] = si[pidx<IT>(e)];
+            nt_store(&out[d0 + e], from_ord<U, FL>(sk[pidx<IT>(e)], flip));
+            if (IDX) nt_store(&iout[d0 + e], si[pidx<IT>(e)]);
         }
     }
 }

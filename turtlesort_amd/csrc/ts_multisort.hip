@@ -58,8 +58,8 @@ __global__ void __launch_bounds__(NT) ms_kernel(const U *__restrict__ in, U *__r
         for (int r = 0; r < ITEMS; r++) {
             const uint32_t i = threadIdx.x + (uint32_t)r * NT;
             if (i < arr_size) {
-                out[base + i] = from_ord<U, FL>(sk[pidx<ITEMS>(i)], flip);
-                if (IDX) iout[base + i] = si[pidx<ITEMS>(i)];
+                nt_store(&out[base + i], from_ord<U, FL>(sk[pidx<ITEMS>(i)], flip));
+                if (IDX) nt_store(&iout[base + i], si[pidx<ITEMS>(i)]);
             }
         }
         return;
@@ -102,8 +102,8 @@ __global__ void __launch_bounds__(NT) ms_kernel(const U *__restrict__ in, U *__r
         const uint32_t q = s >> lg, i = s & (n_eff - 1);
         if (q < nar && i < arr_size) {
             const size_t g = base + (size_t)q * arr_size + i;
-            out[g] = from_ord<U, FL>(sk[pidx<ITEMS>(s)], flip);
-            if (IDX) iout[g] = si[pidx<ITEMS>(s)];
+            nt_store(&out[g], from_ord<U, FL>(sk[pidx<ITEMS>(s)], flip));
+            if (IDX) nt_store(&iout[g], si[pidx<ITEMS>(s)]);
         }
     }
 }
@@ -221,8 +221,8 @@ __global__ void __launch_bounds__(NT) ms_small(const U *__restrict__ in, U *__re
         const uint32_t e = t + r * NT;
         if (e < ne) {
             const uint32_t q = (uint32_t)(((uint64_t)e * inv) >> 32), j = e - q * A;
-            out[base + e] = from_ord<U, FL>((U)sk[q * P + j], flip);
-            if (IDX) iout[base + e] = si[q * P + j];
+            nt_store(&out[base + e], from_ord<U, FL>((U)sk[q * P + j], flip));
+            if (IDX) nt_store(&iout[base + e], si[q * P + j]);
         }
     }
 }
