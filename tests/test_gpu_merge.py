@@ -71,3 +71,34 @@ def test_merge_full_size_int32_idx(torch_dev):
     # the stable sort of x puts the first run's equal keys first, as the merge does
     assert torch.equal(ko, ref_k)
     assert torch.equal(io.long(), ref_i)
+
+
+SHORT_TAIL = [(4096 + 3, 0), (3, 4096), (64 * 4096 - 4, 5), (100, 2 * 4096 - 93), (129 * 4096 - 1000, 1003),
+              (40_000, 64 * 4096 + 6 - 40_000)]
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.int16, np.int32, np.float32, np.uint64])
+def test_merge_short_last_tile(orc, torch_dev, dtype):
+    """outputs whose last tile of 4096 holds 1..7 keys (fewer than 8 bytes
+    for narrow keys): the merge writes nothing past the output, and the
+    crossings of the 64-boundary blocks on either side of such a tile are
+    right"""
+    torch = torch_dev
+    dt = np.dtype(dtype)
+    for i, (na, nb) in enumerate(SHORT_TAIL):
+        n = na + nb
+        assert 1 <= n % 4096 <= 7
+        print(f"case merge short tail dtype={dt} na={na} nb={nb}", flush=True)
+        a, b = make_runs(dt, na, nb, 900 + i, 4)
+        ia = np.arange(na, dtype=np.int32)
+        ib = np.arange(na, na + nb, dtype=np.int32)
+        ek, ei = orc.merge_ranks(a, ia, b, ib)
+        ta, tb = h2d(torch, bits(a)), h2d(torch, bits(b))
+        tia, tib = h2d(torch, ia), h2d(torch, ib)
+        kfull = torch.full((n + 64,), 5, dtype=ta.dtype, device="cuda")
+        ifull = torch.full((n + 64,), -9, dtype=torch.int32, device="cuda")
+        merge_device(ta, tb, kfull[:n], tia, tib, ifull[:n], dtype=dt)
+        np.testing.assert_array_equal(d2h(torch, kfull[:n]).view(dt), ek)
+        np.testing.assert_array_equal(d2h(torch, ifull[:n]), ei)
+        assert bool((kfull[n:] == 5).all()), "keys written past the output"
+        assert bool((ifull[n:] == -9).all()), "index written past the output"

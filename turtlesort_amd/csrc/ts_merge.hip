@@ -13,6 +13,10 @@
 // search: each round every lane tests one evenly spaced candidate and a
 // ballot keeps the interval between the last passing and the first failing
 // candidate, so a run of 2^26 keys takes 5 rounds of loads instead of 26.
+// Every 64th boundary is searched first over the whole runs; the crossing
+// only grows with the diagonal, and so does the count of run-2 elements, so
+// the boundaries in between search the interval their two neighbours leave
+// (at most 64 tiles wide: 3 rounds, about half the scattered line fetches).
 // The tile's slices of both runs are loaded into LDS with coalesced loads,
 // every thread merges ITEMS outputs from its own diagonal (a binary search
 // in LDS, then a serial stable merge), and the tile is written back through
@@ -27,13 +31,16 @@ constexpr int MERGE_NT = 256;
 constexpr int MERGE_ITEMS = 16;
 constexpr uint32_t MERGE_TILE = MERGE_NT * MERGE_ITEMS;
 
+// boundaries between two searched over the whole runs
+constexpr uint64_t CORANK_STRIDE = 64;
+
 // Merge-path crossing of diagonal d (run-1 elements among the first d
-// outputs), searched by one wave: the smallest i in [max(0, d - nb),
-// min(d, na)] with !(a[i] <= b[d - 1 - i]), or the upper end if none.
+// outputs), searched by one wave: the smallest i in [lo, hi] with
+// !(a[i] <= b[d - 1 - i]), or hi if none.  [lo, hi] holds the crossing and
+// lies inside [max(0, d - nb), min(d, na)], where both loads are in range.
 template <typename U, bool FL>
-__device__ __forceinline__ uint64_t corank_wave(const U *a, uint64_t na, const U *b, uint64_t nb, uint64_t d, U flip)
+__device__ __forceinline__ uint64_t corank_wave(const U *a, const U *b, uint64_t d, uint64_t lo, uint64_t hi, U flip)
 {
-    uint64_t lo = d > nb ? d - nb : 0, hi = d < na ? d : na;
     const uint32_t l = lane();
     while (hi > lo) {   // wave-uniform
         const uint64_t len = hi - lo;
@@ -63,22 +70,46 @@ __device__ __forceinline__ uint64_t corank_wave(const U *a, uint64_t na, const U
 }
 
 // The crossings of the tile boundaries, found before the merge by one wave
-// each, all at once, so that no merge workgroup starts with a chain of
-// dependent loads.  The crossing of boundary k (diagonal k*TILE) is kept in
-// the output itself, where only the two workgroups that need it read it
-// before they write anything: the first 8 bytes of tile k and the last 8
-// bytes of tile k-1 (a tile's keys take TILE * sizeof(U) >= 4096 bytes).
-template <typename U, bool FL>
+// each, so that no merge workgroup starts with a chain of dependent loads.
+// The crossing of boundary k (diagonal k*TILE) is kept in the output
+// itself, where only the two workgroups that need it read it before they
+// write anything: the first 8 bytes of tile k and the last 8 bytes of tile
+// k-1 (a tile's keys take TILE * sizeof(U) >= 4096 bytes; only the last
+// tile can be shorter).
+// FINE = false: boundaries CORANK_STRIDE, 2*CORANK_STRIDE, ... over the
+// whole runs.  FINE: the others; wave w takes offset w % (CORANK_STRIDE-1)
+// + 1 after boundary K0 = (w / (CORANK_STRIDE-1)) * CORANK_STRIDE and
+// searches between the crossings c0 of K0 and c1 of K1 = K0 +
+// CORANK_STRIDE (the end of both runs past the last tile): with diagonals
+// D0 < d < D1, c0 <= c(d) <= c1 and D0 - c0 <= d - c(d) <= D1 - c1.
+template <typename U, bool FL, bool FINE>
 __global__ void __launch_bounds__(256) k_merge_corank(const U *__restrict__ a, uint64_t na, const U *__restrict__ b,
                                                       uint64_t nb, unsigned char *__restrict__ out, uint64_t ntiles,
                                                       U flip)
 {
-    const uint64_t k = 1 + (uint64_t)blockIdx.x * (256 / WAVE) + threadIdx.x / WAVE;   // boundary 1 .. ntiles-1
+    const uint64_t w = (uint64_t)blockIdx.x * (256 / WAVE) + threadIdx.x / WAVE;
+    const uint64_t k0 = FINE ? (w / (CORANK_STRIDE - 1)) * CORANK_STRIDE : 0;
+    const uint64_t k = FINE ? k0 + w % (CORANK_STRIDE - 1) + 1 : (w + 1) * CORANK_STRIDE;
     if (k >= ntiles) return;   // wave-uniform
     const uint64_t d = k * MERGE_TILE;
-    const uint64_t c = corank_wave<U, FL>(a, na, b, nb, d, flip);
+    uint64_t lo = d > nb ? d - nb : 0, hi = d < na ? d : na;
+    if constexpr (FINE) {
+        const uint64_t k1 = k0 + CORANK_STRIDE;
+        const uint64_t D0 = k0 * MERGE_TILE, D1 = k1 < ntiles ? k1 * MERGE_TILE : na + nb;
+        // (read from the copies in the last 8 bytes of the tiles before
+        // them, which are whole tiles)
+        const uint64_t c0 = k0 == 0 ? 0ull : *(const uint64_t *)(out + D0 * sizeof(U) - 8);
+        const uint64_t c1 = k1 < ntiles ? *(const uint64_t *)(out + D1 * sizeof(U) - 8) : na;
+        lo = c0 > lo ? c0 : lo;
+        if (d + c1 > D1 && d + c1 - D1 > lo) lo = d + c1 - D1;
+        hi = c1 < hi ? c1 : hi;
+        hi = d - D0 + c0 < hi ? d - D0 + c0 : hi;
+    }
+    const uint64_t c = corank_wave<U, FL>(a, b, d, lo, hi, flip);
     if (lane() == 0) {
-        *(uint64_t *)(out + d * sizeof(U)) = c;
+        // (a last tile of fewer than 8 bytes has no room for its copy: its
+        // workgroup searches the crossing itself)
+        if ((na + nb - d) * sizeof(U) >= 8) *(uint64_t *)(out + d * sizeof(U)) = c;
         *(uint64_t *)(out + d * sizeof(U) - 8) = c;
     }
 }
@@ -97,13 +128,19 @@ __global__ void __launch_bounds__(MERGE_NT) k_merge(const U *__restrict__ a, con
     const uint64_t d0 = (uint64_t)blockIdx.x * T;
     const uint64_t d1 = d0 + T < ntot ? d0 + T : ntot;
     const uint32_t tid = threadIdx.x;
-    if (tid == 0) {
-        // the crossings k_merge_corank left in this tile (read before the
-        // tile is written)
-        const unsigned char *ob = (const unsigned char *)out;
+    // the crossings k_merge_corank left in this tile (read before the tile
+    // is written); a last tile of fewer than 8 bytes has no copy of its
+    // first crossing, which wave 0 searches instead
+    const unsigned char *ob = (const unsigned char *)out;
+    if (d0 != 0 && (ntot - d0) * sizeof(U) < 8) {
+        if (tid < (uint32_t)WAVE) {   // wave 0, every lane
+            const uint64_t c = corank_wave<U, FL>(a, b, d0, d0 > nb ? d0 - nb : 0ull, d0 < na ? d0 : na, flip);
+            if (tid == 0) cor[0] = c;
+        }
+    } else if (tid == 0) {
         cor[0] = d0 == 0 ? 0ull : *(const uint64_t *)(ob + d0 * sizeof(U));
-        cor[1] = d1 == ntot ? na : *(const uint64_t *)(ob + d1 * sizeof(U) - 8);
     }
+    if (tid == 0) cor[1] = d1 == ntot ? na : *(const uint64_t *)(ob + d1 * sizeof(U) - 8);
     block_sync();
     const uint64_t a0 = cor[0], a1 = cor[1];
     const uint64_t b0 = d0 - a0, b1 = d1 - a1;
@@ -191,9 +228,15 @@ static hipError_t merge_launch(uint64_t flip, const void *a, const uint32_t *ia,
     if (grid > 0x7fffffffull) return hipErrorInvalidValue;
     static_assert(MERGE_TILE * sizeof(U) >= 16, "two crossings fit in a tile");
     if (grid > 1) {
-        const uint64_t nb_wg = (grid - 1 + 3) / 4;
-        hipLaunchKernelGGL((k_merge_corank<U, FL>), dim3((uint32_t)nb_wg), dim3(256), 0, st, (const U *)a, na,
-                           (const U *)b, nb, (unsigned char *)out, grid, (U)flip);
+        // the coarse boundaries, then the rest between them (four waves per
+        // workgroup, one boundary each)
+        const uint64_t ncoarse = (grid - 1) / CORANK_STRIDE;
+        if (ncoarse)
+            hipLaunchKernelGGL((k_merge_corank<U, FL, false>), dim3((uint32_t)((ncoarse + 3) / 4)), dim3(256), 0, st,
+                               (const U *)a, na, (const U *)b, nb, (unsigned char *)out, grid, (U)flip);
+        const uint64_t nfine = (grid + CORANK_STRIDE - 1) / CORANK_STRIDE * (CORANK_STRIDE - 1);
+        hipLaunchKernelGGL((k_merge_corank<U, FL, true>), dim3((uint32_t)((nfine + 3) / 4)), dim3(256), 0, st,
+                           (const U *)a, na, (const U *)b, nb, (unsigned char *)out, grid, (U)flip);
     }
     if (iout)
         hipLaunchKernelGGL((k_merge<U, FL, true>), dim3((uint32_t)grid), dim3(MERGE_NT), 0, st, (const U *)a, ia, na,
