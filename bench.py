@@ -28,6 +28,8 @@ end_to_end = the same algorithmic bytes of all passes over the whole sort's
              (HIP events around each timed sort; median and min beside it)
 cpu_baseline = the oracle's restatement of the reference algorithm (one
              core) on a bounded sample of the same distribution
+library_sort = torch.sort(stable=True) (rocPRIM's radix sort) of the same
+             64M keys, timed with events, as context beside value
 """
 import argparse
 import json
@@ -540,6 +542,26 @@ def main():
                               "hbm_frac": round(b2 / (a2 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                               "passes_equiv": round(b2 / floor_bytes, 3), "valid": ok2}
         out["c2_variants"] = variants
+        # the vendor library on the same keys, as context: torch.sort(stable=True)
+        # (rocPRIM's radix sort on ROCm) returns the sorted keys and int64
+        # positions, so it writes 4 bytes more per key than the index leg here
+        fill_keys(kin, wl.SEED_C2_UNIFORM, wl.SHIFT_31BIT)
+        torch.cuda.synchronize()
+        torch.sort(kin, stable=True)
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record()
+        for _ in range(args.steps):
+            lv, li = torch.sort(kin, stable=True)
+        ev[1].record()
+        torch.cuda.synchronize()
+        lib_ms = ev[0].elapsed_time(ev[1]) / args.steps
+        out["library_sort"] = {
+            "workload": "torch.sort(stable=True) of the same 64M int32 keys: sorted keys + int64 positions",
+            "ms_per_sort": round(lib_ms, 4), "Gkeys_s": round(n / (lib_ms * 1e-3) / 1e9, 4),
+            "this_vs_library": round(value / (n / (lib_ms * 1e-3) / 1e9), 2),
+            "valid": bool(torch.equal(lv, kin[li])) and bool((lv[1:] >= lv[:-1]).all())}
+        del lv, li
 
     # the host-array API (StartSorting/Sync with pinned staging): PCIe included,
     # reported beside the device-resident value, never as it
