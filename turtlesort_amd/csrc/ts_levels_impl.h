@@ -941,7 +941,16 @@ struct LevelK {
         const uint32_t ntot = p.ctrl->tile_alloc < p.tilecap ? p.ctrl->tile_alloc : p.tilecap;
         const uint32_t G = p.tiles_per_wg;
         uint64_t nbytes = 0;
-        for (uint32_t c0 = blockIdx.x * G; c0 < ntot; c0 += gridDim.x * G) {
+        // XCD-aware order: workgroups are dispatched round-robin over the 8
+        // XCDs (workgroup b runs on XCD b % 8), and each XCD has its own L2.
+        // The sweep's chunks are renumbered (an 8 x gridDim.x/8 transpose) so
+        // that neighbouring chunks go to workgroups of one XCD: a bin's run
+        // from one tile and the next tile's run of that bin share their edge
+        // lines, which then fill up in one L2 instead of being written back
+        // as partial lines from two
+        uint32_t b = blockIdx.x;
+        if (gridDim.x % 8 == 0) b = (b % 8) * (gridDim.x / 8) + b / 8;
+        for (uint32_t c0 = b * G; c0 < ntot; c0 += gridDim.x * G) {
             const uint32_t c1 = c0 + G < ntot ? c0 + G : ntot;
             for (uint32_t g = c0; g < c1; g++) {
                 uint32_t sid, t;
