@@ -29,9 +29,19 @@ hipError_t engine_clear_classes(const Params &p, hipStream_t st)
 
 // Roots of a batch of segments [off[s], off[s+1]) (off == nullptr: the
 // single segment [0, single_len)): large ones become level-0 segments,
-// small ones LDS tasks.  Appends are aggregated per wave.
-__global__ void k_roots(Params p, const uint32_t *off, uint32_t nseg, uint32_t single_len)
+// small ones LDS tasks.  Appends are aggregated per wave.  reset: the launch
+// is one workgroup, which first clears the control block itself (instead of
+// a k_reset launch before it).
+__global__ void k_roots(Params p, const uint32_t *off, uint32_t nseg, uint32_t single_len, uint32_t reset)
 {
+    if (reset) {   // grid-uniform
+        uint32_t *w = (uint32_t *)p.ctrl;
+        for (uint32_t i = threadIdx.x; i < sizeof(Ctrl) / 4; i += blockDim.x) w[i] = 0;
+        // every thread's clears are complete at device scope before any wave
+        // of the workgroup appends through device-scope atomics on the block
+        __threadfence();
+        __syncthreads();
+    }
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t beg = 0, len = 0;
     if (s < nseg) {

@@ -230,7 +230,7 @@ __device__ __forceinline__ void classify_n(const U (&x)[N], uint32_t (&bin)[N], 
 
 // ---- set-up kernels ------------------------------------------------------------------
 __global__ void k_reset(Ctrl *c);
-__global__ void k_roots(Params p, const uint32_t *off, uint32_t nseg, uint32_t single_len);
+__global__ void k_roots(Params p, const uint32_t *off, uint32_t nseg, uint32_t single_len, uint32_t reset);
 
 // LDS-sort plan from the per-class task counts: first slot of every class
 // in the class-ordered task list and first workgroup step of every class
@@ -2099,9 +2099,16 @@ hipError_t run_levels(const Params &p, int cus, int wg_per_cu, const uint32_t *d
         if (tm) tm->end(kind, st);            \
     } while (0)
     if (first) {
-        TS_LAUNCH(K_RESET, k_reset, dim3(1), dim3(256), 0, st, p.ctrl);
-        if (nseg > 0)
-            TS_LAUNCH(K_ROOTS, k_roots, dim3((nseg + 255) / 256), dim3(256), 0, st, p, dev_off, nseg, single_len);
+        // up to 256 roots (every plain sort): one launch clears the control
+        // block and appends the roots
+        if (nseg > 0 && nseg <= 256) {
+            TS_LAUNCH(K_ROOTS, k_roots, dim3(1), dim3(256), 0, st, p, dev_off, nseg, single_len, 1u);
+        } else {
+            TS_LAUNCH(K_RESET, k_reset, dim3(1), dim3(256), 0, st, p.ctrl);
+            if (nseg > 0)
+                TS_LAUNCH(K_ROOTS, k_roots, dim3((nseg + 255) / 256), dim3(256), 0, st, p, dev_off, nseg, single_len,
+                          0u);
+        }
     }
     for (uint32_t L = L0; L < L0 + nlev && L < (uint32_t)MAX_LEVELS; L++) {
         TS_LAUNCH(K_PIVOT, (k_pivot<KT, IDX, S>), dim3(g_pivot), dim3(512), 0, st, p, L);
